@@ -1,0 +1,159 @@
+"""Benchmark: FP64 zone-updates/s of the fused OpenMHD step on one MI355X.
+
+Workload (the configuration the metric is quoted on): 2D Kelvin-Helmholtz
+layer, 4096 x 4096 cells, FP64, x periodic, y walls, minmod + HLLD + GLM +
+TVD-RK2, CFL 0.4.  A "step" is one full TVD-RK2 step of the whole grid (two
+fused launches + the CFL reduction); the state is resident in HBM before the
+timed region starts.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BYTES_PER_ZONE_UPDATE = 360  # 9 doubles: read U^n, write U*, read U^n + U*, write U^{n+1}
+BYTES_STAGE = {1: 144, 2: 216}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse_traffic():
+    """HBM bytes per stage-2 launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    f = REPO / "profiles" / "pmc_sweep2d_kh4096.txt"
+    if not f.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch_stage2:\s*([0-9.eE+]+)", f.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n: int, sample_steps: int):
+    """The reference CPU build (oracle/_ref, the project's Fortran driver over the
+    reference's own routines, OpenMP) timed on this host on a bounded sample of
+    the same workload; falls back to the C restatement if _ref is absent."""
+    threads = min(16, os.cpu_count() or 1)
+    drv = REPO / "oracle" / "_ref" / "main2d_ref_omp"
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    if drv.exists():
+        cmd = [str(drv), "--kh", str(n), str(sample_steps + 1), "--time"]
+        log("cpu baseline:", " ".join(cmd))
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        m = re.search(r"zone_updates_per_s\s+([0-9.eE+]+)", out.stdout)
+        if out.returncode == 0 and m:
+            return {"value": float(m.group(1)), "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+                    "sample": f"KH {n}x{n} minmod, steps 2..{sample_steps + 1} of a {sample_steps + 1}-step run, "
+                              f"reference routines built with flang -O2 -fopenmp, {threads} threads"}
+        log("reference driver failed:", out.stdout[-500:], out.stderr[-500:])
+    # C restatement (port)
+    sys.path.insert(0, str(REPO / "tests"))
+    from omhd_testlib import Oracle, make_problem
+
+    from openmhd_amd.problems import kelvin_helmholtz
+
+    U, dx = kelvin_helmholtz(n, n, seed=1)
+    orc = Oracle()
+    p = make_problem(n + 2, n + 2, dx=dx, cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"))
+    orc.bc_u(U, p)
+    orc.run(p, U, 1)
+    t0 = time.perf_counter()
+    orc.run(p, U, sample_steps)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * sample_steps / dt, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"KH {n}x{n} minmod, {sample_steps} steps after one untimed step, C restatement "
+                      f"(gcc -O2 -fopenmp, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
+    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py: this build runs on one GPU only (the grid is one domain on one MI355X); "
+                 "multi-GPU sharding is a design note, not built -- run with --gpus 1")
+
+    import torch
+
+    sys.path.insert(0, str(REPO))
+    from openmhd_amd import Solver2D
+    from openmhd_amd.problems import kelvin_helmholtz
+
+    torch.cuda.set_device(0)
+    n = a.n
+    log(f"building KH {n}x{n} initial state on the host")
+    U, dx = kelvin_helmholtz(n, n, seed=1)
+    s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows)
+    s.upload(U)
+    del U
+    log(f"warmup {a.warmup} steps")
+    s.step(a.warmup)
+    s.set_timing(True)
+    torch.cuda.synchronize()
+    log(f"timing {a.steps} steps")
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms, launches = s.timing()
+    elapsed = t1 - t0
+    cells = n * n
+    zups = cells * a.steps / elapsed
+    ms_step = 1e3 * elapsed / a.steps
+    st = [ms[i] / max(1, launches[i]) for i in range(2)]  # average launch duration, ms
+    dom = 1 if st[1] >= st[0] else 0
+    achieved = BYTES_STAGE[dom + 1] * cells / (st[dom] * 1e-3) / 1e9
+    traffic = parse_traffic()
+    log(f"{ms_step:.3f} ms/step, {zups / 1e9:.3f} G zone-updates/s; stage1 {st[0]:.3f} ms, stage2 {st[1]:.3f} ms")
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(n, a.cpu_sample_steps)
+        except Exception as e:  # the baseline is reported, never the measured value
+            log("cpu baseline failed:", repr(e))
+    out = {
+        "metric": "FP64 zone-updates/s & %HBM peak; batched and adjoint time vs plain GPU step",
+        "value": zups,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded KH initial state generated on the host)",
+        "config": {"workload": f"2D Kelvin-Helmholtz {n}x{n} FP64, x periodic, y walls, minmod + HLLD + GLM + "
+                               f"TVD-RK2, CFL 0.4", "nx": n, "ny": n, "limiter": "minmod", "flux": "HLLD",
+                   "time": "TVD-RK2", "parallelism": "single GPU, one domain"},
+        "hbm_frac_step": zups * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": f"k_sweep2d stage {dom + 1}", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_ms": st[dom], "algorithmic_bytes_per_launch": BYTES_STAGE[dom + 1] * cells},
+        "cpu_baseline": cpu,
+    }
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+/*
+ * omhd.h -- C ABI of libomhd, the MI355X (gfx950) implementation of
+ * OpenMHD's finite-volume time step.
+ *
+ * Two ways in:
+ *
+ *  1. Drop-in entry points, one per reference subroutine, with the reference's
+ *     argument lists (Fortran column-major arrays including the single ghost
+ *     layer, extents ix, jx[, kx]).  A Fortran caller reaches them through the
+ *     iso_c_binding shims in openmhd_amd/fortran/omhd_shims.f90, which keep the
+ *     reference names (u2v, limiter, flux_solver, ...), so a reference-style
+ *     main.f90 links against them unchanged.  Array pointers may be host or
+ *     device memory: host arrays are staged through device buffers per call,
+ *     device arrays are used in place.  Every call returns when its results
+ *     are in the caller's arrays.
+ *
+ *  2. The resident step: a handle owns the state in HBM (structure of arrays,
+ *     interior cells only) and omhd_step() advances it by whole TVD-RK2 steps
+ *     with the fused sweep kernels.  This is the fast path.
+ *
+ * Every function returns OMHD_OK (0) or an OMHD_ERR_* code; omhd_last_error()
+ * gives the cell and value behind the last failure on the calling thread.
+ * The reference reports these conditions by printing and calling STOP
+ * (u2v.f90:34-43, set_dt.f90:55-65, limiter.f90:310-312).
+ */
+#ifndef OMHD_H
+#define OMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define OMHD_OK 0
+#define OMHD_ERR_PRESSURE 1   /* p <= 0 somewhere (u2v.f90:34-38)          */
+#define OMHD_ERR_DENSITY 2    /* rho <= 0 somewhere (u2v.f90:39-43)        */
+#define OMHD_ERR_DT 3         /* dt < 1e-7 or not finite (set_dt.f90:55-65) */
+#define OMHD_ERR_DIR 4        /* dir not in 1..3 (or 3 on a 2D grid)       */
+#define OMHD_ERR_TYPE 5       /* unknown limiter / flux / time type        */
+#define OMHD_ERR_EXTENT 6     /* extents too small or inconsistent         */
+#define OMHD_ERR_HIP 7        /* HIP runtime failure (message in last_error) */
+#define OMHD_ERR_ARG 8        /* null pointer or bad argument              */
+
+typedef struct omhd_error_info {
+  int32_t code;
+  int32_t i, j, k;   /* 1-based Fortran cell index incl. ghosts, when known */
+  double value;      /* offending p, rho, dt, ...                          */
+  char msg[160];
+} omhd_error_info;
+
+/* Last error recorded on the calling thread (cleared by a successful call). */
+int omhd_last_error(omhd_error_info *out);
+
+/* ---- boundary description ---------------------------------------------- */
+#define OMHD_BC_PERIODIC 0
+#define OMHD_BC_MIRROR 1      /* ghost = mirror of the first interior cell
+                                 with the slots in `sign` negated; face state
+                                 = mirror of the inside face state          */
+
+/* ---- drop-in entry points (reference subroutine interfaces) ------------- */
+
+/* u2v(U,V,ix,jx[,kx]) -- 2D_basic/u2v.f90:1, 3D_basic/u2v.f90:1.
+ * U(ix,jx,kx,9) -> V(ix,jx,kx,4) over all cells; positivity check. */
+int omhd_u2v(const double *U, double *V, int32_t ix, int32_t jx, int32_t kx);
+
+/* limiter(wk,wL,wR,ix,jx[,kx],dir,type) -- 2D_basic/limiter.f90:1,
+ * 3D_basic/limiter.f90:1.  One variable plane, one direction, types 0..4. */
+int omhd_limiter(const double *wk, double *wL, double *wR, int32_t ix, int32_t jx, int32_t kx, int32_t dir,
+                 int32_t type);
+
+/* flux_solver(F,VL,VR,ch,ix,jx[,kx],dir,type) -- 2D_basic/flux_solver.f90:1,
+ * 3D_basic/flux_solver.f90:1.  type 3 (HLLD).  F is zeroed first, as in the
+ * CPU build (flux_solver.f90:57). */
+int omhd_flux_solver(double *F, const double *VL, const double *VR, double ch, int32_t ix, int32_t jx, int32_t kx,
+                     int32_t dir, int32_t type);
+
+/* glm_ss2(U,ch,dt,ix,jx[,kx]) -- 2D_basic/glm_ss2.f90:1.  psi *= exp(-dt*ch/(2*0.2))
+ * over the whole array (ghosts included); the factor is computed on the host. */
+int omhd_glm_ss2(double *U, double ch, double dt, int32_t ix, int32_t jx, int32_t kx);
+
+/* rk_tvd21 / rk_tvd22 / rk_std21 / rk_std22 -- 2D_basic/rk.f90:9,42,73,106,
+ * 3D_basic/rk.f90:9,46,81,118.  mode 0..3 in that order.  wH == NULL for 2D.
+ * For rk_std22 (mode 3) wk1 is not used. */
+int omhd_rk(int32_t mode, double *wk, double *wk1, const double *wF, const double *wG, const double *wH, double dt,
+            double dx, int32_t ix, int32_t jx, int32_t kx);
+
+/* set_dt(U,V,vmax,dt,dx,cfl,ix,jx[,kx]) -- 2D_basic/set_dt.f90:1,
+ * 3D_basic/set_dt.f90:1.  ndim selects the 2D (x,y) or 3D (x,y,z) speeds. */
+int omhd_set_dt(const double *U, const double *V, double *vmax, double *dt, double dx, double cfl, int32_t ix,
+                int32_t jx, int32_t kx, int32_t ndim);
+
+/* bc(U,ix,jx[,kx]) / bc_for_U(U,ix,jx[,kx]) -- 2D_basic/bc.f90:1,
+ * 2D_KH/bc.f90:6, 3D_basic/bc.f90:1.  bc_type[6] per face (x-,x+,y-,y+,z-,z+),
+ * sign[6*9] per face and slot (+1 / -1) for OMHD_BC_MIRROR faces. */
+int omhd_bc_u(double *U, int32_t ix, int32_t jx, int32_t kx, const int32_t *bc_type, const double *sign);
+
+/* Interface-state face patch along dir: the inline periodic patch of
+ * 2D_basic/main.f90:101-102,119-120 and bc_for_F/G of 2D_KH/bc.f90:42,55. */
+int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int32_t dir, int32_t bc_type,
+                 const double *sign_lo, const double *sign_hi);
+
+/* ---- resident step ------------------------------------------------------ */
+
+typedef struct omhd_config {
+  int32_t ndim;         /* 2 (3D is not built yet)                          */
+  int32_t nx, ny, nz;   /* interior cells (the reference's ix-2, jx-2, kx-2) */
+  double dx;            /* cell size, dx == dy (rk.f90:4-5)                 */
+  double cfl;           /* CFL number (main.f90:15)                         */
+  int32_t lm_type;      /* 0 flat, 1 minmod, 2 MC, 3 van Leer, 4 Koren      */
+  int32_t flux_type;    /* 3 HLLD                                           */
+  int32_t time_type;    /* 0 TVD-RK2, 1 RK2 (main.f90:20-21)                */
+  int32_t bc_type[6];   /* OMHD_BC_* per face: x-, x+, y-, y+, z-, z+       */
+  double bc_sign[54];   /* mirror sign per face and slot (6 x 9)            */
+  int32_t rows_per_wave;/* 0: default; sweep chunk height (tuning)          */
+  int32_t reserved[7];
+} omhd_config;
+
+typedef struct omhd_handle omhd_handle;
+
+/* Allocate the device state for cfg on the current HIP device. */
+int omhd_create(const omhd_config *cfg, omhd_handle **h);
+
+/* Load U(ix,jx[,kx],9) (Fortran layout, ghosts ignored) and compute the first
+ * CFL speed, as the reference does before its loop (main.f90:38-40). */
+int omhd_upload(omhd_handle *h, const double *U);
+
+/* Advance nsteps whole steps (main.f90:56-188 minus I/O).  t (optional) is
+ * advanced by each dt; dt_trace/ch_trace (optional, nsteps doubles) receive
+ * each step's dt and ch.  Stops at the first error, which is returned with the
+ * number of completed steps in *done (optional). */
+int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done);
+
+/* Copy the state back as U(ix,jx[,kx],9) with ghosts filled by the boundary
+ * conditions, and optionally V(ix,jx[,kx],4) (primitives of every cell). */
+int omhd_download(omhd_handle *h, double *U, double *V);
+
+/* Device pointer of the state (SoA, 9 planes) and its layout, for callers that
+ * keep the data on the GPU (the PyTorch binding, the benchmark). */
+int omhd_state_layout(omhd_handle *h, double **dev_state, int64_t *plane_stride, int32_t *row_pitch);
+
+/* HIP stream the handle's kernels run on (as void*: a hipStream_t). */
+void *omhd_stream(omhd_handle *h);
+
+int omhd_destroy(omhd_handle *h);
+
+/* Per-stage kernel timing with HIP events recorded on the handle's stream
+ * around each stage launch (no counterpart in the reference, which has no
+ * timers).  Enabling resets the accumulators; stage_ms[2] / launches[2] get
+ * the summed milliseconds and launch counts of stage 1 and stage 2. */
+int omhd_set_timing(omhd_handle *h, int32_t enable);
+int omhd_get_timing(omhd_handle *h, double *stage_ms, int64_t *launches);
+
+/* Library version and the gfx target it was built for. */
+const char *omhd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMHD_H */

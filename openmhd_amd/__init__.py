@@ -1,0 +1,15 @@
+"""openmhd_amd -- OpenMHD's finite-volume time step on MI355X (gfx950).
+
+The compute is libomhd.so (hand-written HIP, built in-tree); this package is a
+thin ctypes layer over its C ABI (include/omhd.h):
+
+  openmhd_amd.api      the reference subroutines (u2v, limiter, flux_solver,
+                       glm_ss2, rk_tvd21/22, rk_std21/22, set_dt, bc, bc_for_*)
+  openmhd_amd.solver   Solver2D: resident state, fused whole-step kernels
+  openmhd_amd.problems initial states of the benchmark configurations
+"""
+from ._lib import LIB_PATH, OMHDError, lib  # noqa: F401  (raises ImportError if libomhd is not built)
+from . import api, problems, solver  # noqa: F401
+from .solver import Solver2D  # noqa: F401
+
+__all__ = ["api", "solver", "problems", "Solver2D", "OMHDError", "LIB_PATH"]

@@ -1,0 +1,544 @@
+// omhd_dropin.hip -- one HIP kernel per reference subroutine, on the
+// reference's own Fortran array layout (column-major, one ghost layer), behind
+// the drop-in C entry points of include/omhd.h.  These exist so that a
+// reference-style driver can call u2v/limiter/flux_solver/... one by one and
+// get the reference's results; the fused kernels in omhd_step.hip are the
+// fast path.  One lane per cell (or interface).
+#include <math.h>
+#include <mutex>
+
+#include "omhd_device.h"
+#include "omhd_host.h"
+
+namespace omhd {
+
+omhd_error_info &tls_error() {
+  static thread_local omhd_error_info e{};
+  return e;
+}
+
+hipStream_t dropin_stream() {
+  static hipStream_t s = nullptr;
+  static std::once_flag once;
+  std::call_once(once, [] { (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking); });
+  return s;
+}
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ---------------------------------------------------------------------------
+// u2v over all cells (u2v.f90:20-31); flags |= 1 for p <= 0, |= 2 for rho <= 0
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_u2v(const double *__restrict__ U, double *__restrict__ V, size_t n, int *flags) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double u[NV], w[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) u[m] = U[m * n + c];
+  cons_to_prim(u, w);
+#pragma unroll
+  for (int m = 0; m < 4; m++) V[m * n + c] = w[m];
+  int f = 0;
+  if (w[3] <= 0) f |= 1;
+  if (u[4] <= 0) f |= 2;
+  if (f) atomicOr(flags, f);
+}
+
+// ---------------------------------------------------------------------------
+// limiter along one direction (limiter.f90).  Lines run along `dir` with
+// stride s and length nl; (a, b) enumerate the transverse ranges.
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_limiter(const double *__restrict__ wk, double *__restrict__ wL, double *__restrict__ wR, int nl,
+                          size_t s, int na, int nb, size_t sa, size_t sb, size_t base) {
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)nl * na * nb;
+  if (gid >= total) return;
+  const int l = (int)(gid % nl);  // 0-based position on the line
+  const size_t r = gid / nl;
+  const int a = (int)(r % na), b = (int)(r / na);
+  const size_t o = base + (size_t)a * sa + (size_t)b * sb;
+  if constexpr (T == 0) {  // limiter.f90:56-70
+    if (l < nl - 1) {
+      wL[o + l * s] = wk[o + l * s];
+      wR[o + l * s] = wk[o + (l + 1) * s];
+    }
+  } else {
+    if (l == 0) {
+      wL[o] = wk[o];
+    } else if (l == nl - 1) {
+      wR[o + (size_t)(nl - 2) * s] = wk[o + (size_t)(nl - 1) * s];
+    } else {
+      double gR, gL;
+      limit<T>(wk[o + (l - 1) * s], wk[o + l * s], wk[o + (l + 1) * s], gR, gL);
+      wR[o + (l - 1) * s] = gR;
+      wL[o + l * s] = gL;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// HLLD flux at every interface of the reference's loop range
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_flux(double *__restrict__ F, const double *__restrict__ VL, const double *__restrict__ VR, double ch,
+                       int ix, int jx, int kx, int i0, int ni, int j0, int nj, int k0, int nk) {
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)ni * nj * nk) return;
+  const int i = i0 + (int)(gid % ni);
+  const size_t r = gid / ni;
+  const int j = j0 + (int)(r % nj), k = k0 + (int)(r / nj);
+  const size_t n = (size_t)ix * jx * kx;
+  const size_t c = (size_t)i + (size_t)ix * ((size_t)j + (size_t)jx * k);
+  double L[NV], R[NV], f[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    L[m] = VL[m * n + c];
+    R[m] = VR[m * n + c];
+  }
+  hlld<D>(L, R, ch, ch * ch, f);
+#pragma unroll
+  for (int m = 0; m < NV; m++) F[m * n + c] = f[m];
+}
+
+__global__ void k_scale(double *__restrict__ a, double f, size_t n) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) a[c] = f * a[c];
+}
+
+// ---------------------------------------------------------------------------
+// RK stages (rk.f90): interior cells, all 9 variables
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rk(int mode, double *__restrict__ wk, double *__restrict__ wk1, const double *__restrict__ wF,
+                     const double *__restrict__ wG, const double *__restrict__ wH, double dtx, int ix, int jx, int kx) {
+  const int ni = ix - 2, nj = jx - 2, nk = wH ? kx - 2 : 1;
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)ni * nj * nk) return;
+  const int i = 1 + (int)(gid % ni);
+  const size_t r = gid / ni;
+  const int j = 1 + (int)(r % nj), k = (wH ? 1 : 0) + (int)(r / nj);
+  const size_t n = (size_t)ix * jx * kx;
+  const size_t c = (size_t)i + (size_t)ix * ((size_t)j + (size_t)jx * k);
+  const size_t sj = ix, sk = (size_t)ix * jx;
+  for (int m = 0; m < NV; m++) {
+    const size_t q = m * n + c;
+    double d = wF[q - 1] - wF[q] + wG[q - sj] - wG[q];
+    if (wH) d = d + wH[q - sk] - wH[q];
+    if (mode == 0 || mode == 2) wk1[q] = wk[q] + dtx * d;
+    else if (mode == 1) wk[q] = 0.5 * (wk[q] + wk1[q] + dtx * d);
+    else wk[q] = wk[q] + dtx * d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set_dt: per-cell fastest speed, wave max, one atomic per wave
+// ---------------------------------------------------------------------------
+template <int ND>
+__global__ __launch_bounds__(256) void k_speed(const double *__restrict__ U, const double *__restrict__ V, int ix, int jx, int kx,
+                        unsigned long long *vmax_bits, int *flags) {
+  const int is = ix > 1 ? 1 : 0, js = jx > 1 ? 1 : 0, ks = kx > 1 ? 1 : 0;
+  const int ni = ix > 2 ? ix - 2 : 1, nj = jx > 2 ? jx - 2 : 1, nk = kx > 2 ? kx - 2 : 1;
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double vt = 0.0;
+  if (gid < (size_t)ni * nj * nk) {
+    const int i = is + (int)(gid % ni);
+    const size_t r = gid / ni;
+    const int j = js + (int)(r % nj), k = ks + (int)(r / nj);
+    const size_t n = (size_t)ix * jx * kx;
+    const size_t c = (size_t)i + (size_t)ix * ((size_t)j + (size_t)jx * k);
+    double u[NV], w[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) u[m] = U[m * n + c];
+#pragma unroll
+    for (int m = 0; m < 4; m++) w[m] = V[m * n + c];
+    vt = fastest_speed<ND>(u, w);
+    if (!(vt >= 0.0)) atomicOr(flags, 4);  // NaN
+  }
+  vt = wave_max(vt);
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax_bits, vt);
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill, one face pair per launch (x, then y, then z: bc.f90 order)
+// ---------------------------------------------------------------------------
+__global__ void k_bc_face_pair(double *__restrict__ U, int ix, int jx, int kx, int dir, int type_lo, int type_hi,
+                               const double sl0, const double sl1, const double sl2, const double sl3, const double sl4,
+                               const double sl5, const double sl6, const double sl7, const double sl8, const double sh0,
+                               const double sh1, const double sh2, const double sh3, const double sh4, const double sh5,
+                               const double sh6, const double sh7, const double sh8) {
+  // transverse plane enumeration
+  const int na = (dir == 1) ? jx : ix, nb = (dir == 3) ? jx : kx;
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)na * nb) return;
+  const int a = (int)(gid % na), b = (int)(gid / na);
+  const size_t n = (size_t)ix * jx * kx;
+  size_t base, s;
+  int nl;
+  if (dir == 1) { base = (size_t)ix * ((size_t)a + (size_t)jx * b); s = 1; nl = ix; }
+  else if (dir == 2) { base = (size_t)a + (size_t)ix * (size_t)jx * b; s = ix; nl = jx; }
+  else { base = (size_t)a + (size_t)ix * b; s = (size_t)ix * jx; nl = kx; }
+  const double sl[NV] = {sl0, sl1, sl2, sl3, sl4, sl5, sl6, sl7, sl8};
+  const double sh[NV] = {sh0, sh1, sh2, sh3, sh4, sh5, sh6, sh7, sh8};
+  for (int m = 0; m < NV; m++) {
+    double *A = U + m * n + base;
+    if (type_lo == OMHD_BC_PERIODIC) {  // bc.f90:9-14 (high side first)
+      A[(size_t)(nl - 1) * s] = A[s];
+      A[0] = A[(size_t)(nl - 2) * s];
+    } else {
+      A[0] = sl[m] * A[s];
+      A[(size_t)(nl - 1) * s] = sh[m] * A[(size_t)(nl - 2) * s];
+    }
+  }
+  (void)type_hi;
+}
+
+__global__ void k_face_patch(double *__restrict__ VL, double *__restrict__ VR, int ix, int jx, int kx, int dir,
+                             int type, const double sl0, const double sl1, const double sl2, const double sl3,
+                             const double sl4, const double sl5, const double sl6, const double sl7, const double sl8,
+                             const double sh0, const double sh1, const double sh2, const double sh3, const double sh4,
+                             const double sh5, const double sh6, const double sh7, const double sh8) {
+  const int na = (dir == 1) ? jx : ix, nb = (dir == 3) ? jx : kx;
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)na * nb) return;
+  const int a = (int)(gid % na), b = (int)(gid / na);
+  const size_t n = (size_t)ix * jx * kx;
+  size_t base, s;
+  int nl;
+  if (dir == 1) { base = (size_t)ix * ((size_t)a + (size_t)jx * b); s = 1; nl = ix; }
+  else if (dir == 2) { base = (size_t)a + (size_t)ix * (size_t)jx * b; s = ix; nl = jx; }
+  else { base = (size_t)a + (size_t)ix * b; s = (size_t)ix * jx; nl = kx; }
+  const double sl[NV] = {sl0, sl1, sl2, sl3, sl4, sl5, sl6, sl7, sl8};
+  const double sh[NV] = {sh0, sh1, sh2, sh3, sh4, sh5, sh6, sh7, sh8};
+  const size_t lo = base, hi = base + (size_t)(nl - 2) * s;
+  for (int m = 0; m < NV; m++) {
+    double *L = VL + m * n, *R = VR + m * n;
+    if (type == OMHD_BC_PERIODIC) {  // main.f90:101-102
+      R[hi] = R[lo];
+      L[lo] = L[hi];
+    } else {  // 2D_KH/bc.f90:61-81
+      L[lo] = sl[m] * R[lo];
+      R[hi] = sh[m] * L[hi];
+    }
+  }
+}
+
+int check_ext(int ix, int jx, int kx) {
+  if (ix < 1 || jx < 1 || kx < 1) return set_error(OMHD_ERR_EXTENT, "non-positive extent", ix, jx, kx);
+  return OMHD_OK;
+}
+
+// 1-based (i,j,k) of a 0-based linear cell index
+void unflatten(size_t c, int ix, int jx, int &i, int &j, int &k) {
+  i = (int)(c % ix) + 1;
+  size_t r = c / ix;
+  j = (int)(r % jx) + 1;
+  k = (int)(r / jx) + 1;
+}
+
+// minloc of plane[0..n) held by the caller (host or device)
+size_t plane_minloc(const double *plane, size_t n, bool dev, double &val) {
+  const double *h = plane;
+  double *tmp = nullptr;
+  if (dev) {
+    tmp = (double *)malloc(n * sizeof(double));
+    (void)hipMemcpy(tmp, plane, n * sizeof(double), hipMemcpyDeviceToHost);
+    h = tmp;
+  }
+  size_t best = 0;
+  for (size_t c = 1; c < n; c++)
+    if (h[c] < h[best]) best = c;
+  val = h[best];
+  free(tmp);
+  return best;
+}
+
+}  // namespace
+}  // namespace omhd
+
+using namespace omhd;
+
+extern "C" int omhd_last_error(omhd_error_info *out) {
+  if (!out) return OMHD_ERR_ARG;
+  *out = tls_error();
+  return OMHD_OK;
+}
+
+extern "C" const char *omhd_version(void) { return "omhd 0.1 (gfx950, fp64, fused column-march sweeps)"; }
+
+extern "C" int omhd_u2v(const double *U, double *V, int32_t ix, int32_t jx, int32_t kx) {
+  if (!U || !V) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged dU, dV;
+  int *flags;
+  if (int rc = dU.init(U, 9 * n, true, false, s)) return rc;
+  if (int rc = dV.init(V, 4 * n, false, true, s)) return rc;
+  OMHD_HIP(hipMallocAsync((void **)&flags, sizeof(int), s));
+  OMHD_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
+  k_u2v<<<nblocks(n), TPB, 0, s>>>(dU.dev, dV.dev, n, flags);
+  OMHD_HIP(hipGetLastError());
+  int hflags = 0;
+  OMHD_HIP(hipMemcpyAsync(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (int rc = dV.finish(s)) return rc;
+  OMHD_HIP(hipStreamSynchronize(s));
+  (void)hipFreeAsync(flags, s);
+  dU.release(s);
+  dV.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  if (hflags & 1) {  // u2v.f90:34-38: report minloc of p
+    double val;
+    size_t c = plane_minloc(V + 3 * n, n, is_device_ptr(V), val);
+    int i, j, k;
+    unflatten(c, ix, jx, i, j, k);
+    return set_error(OMHD_ERR_PRESSURE, "negative pressure", i, j, k, val);
+  }
+  if (hflags & 2) {  // :39-43
+    double val;
+    size_t c = plane_minloc(U + 4 * n, n, is_device_ptr(U), val);
+    int i, j, k;
+    unflatten(c, ix, jx, i, j, k);
+    return set_error(OMHD_ERR_DENSITY, "negative density", i, j, k, val);
+  }
+  return ok();
+}
+
+extern "C" int omhd_limiter(const double *wk, double *wL, double *wR, int32_t ix, int32_t jx, int32_t kx, int32_t dir,
+                            int32_t type) {
+  if (!wk || !wL || !wR) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (type < 0 || type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", type);
+  if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged a, bl, br;
+  if (int rc = a.init(wk, n, true, false, s)) return rc;
+  // outputs are only partly written (limiter.f90 skips the transverse ghosts):
+  // stage them in so untouched entries keep the caller's values
+  if (int rc = bl.init(wL, n, true, true, s)) return rc;
+  if (int rc = br.init(wR, n, true, true, s)) return rc;
+  auto lo = [](int e) { return e < 2 ? e : 2; };
+  auto hi = [](int e) { return e - 1 > 1 ? e - 1 : 1; };
+  int nl, na, nb;
+  size_t st, sa, sb, base;
+  const size_t sj = ix, sk = (size_t)ix * jx;
+  if (dir == 1) {
+    nl = ix; st = 1;
+    na = hi(jx) - lo(jx) + 1; sa = sj;
+    nb = hi(kx) - lo(kx) + 1; sb = sk;
+    base = (size_t)(lo(jx) - 1) * sj + (size_t)(lo(kx) - 1) * sk;
+  } else if (dir == 2) {
+    nl = jx; st = sj;
+    na = hi(ix) - lo(ix) + 1; sa = 1;
+    nb = hi(kx) - lo(kx) + 1; sb = sk;
+    base = (size_t)(lo(ix) - 1) + (size_t)(lo(kx) - 1) * sk;
+  } else {
+    nl = kx; st = sk;
+    na = hi(ix) - lo(ix) + 1; sa = 1;
+    nb = hi(jx) - lo(jx) + 1; sb = sj;
+    base = (size_t)(lo(ix) - 1) + (size_t)(lo(jx) - 1) * sj;
+  }
+  const size_t total = (size_t)nl * na * nb;
+  if (nl >= 2 && total > 0) {
+    switch (type) {
+      case 0: k_limiter<0><<<nblocks(total), TPB, 0, s>>>(a.dev, bl.dev, br.dev, nl, st, na, nb, sa, sb, base); break;
+      case 1: k_limiter<1><<<nblocks(total), TPB, 0, s>>>(a.dev, bl.dev, br.dev, nl, st, na, nb, sa, sb, base); break;
+      case 2: k_limiter<2><<<nblocks(total), TPB, 0, s>>>(a.dev, bl.dev, br.dev, nl, st, na, nb, sa, sb, base); break;
+      case 3: k_limiter<3><<<nblocks(total), TPB, 0, s>>>(a.dev, bl.dev, br.dev, nl, st, na, nb, sa, sb, base); break;
+      default: k_limiter<4><<<nblocks(total), TPB, 0, s>>>(a.dev, bl.dev, br.dev, nl, st, na, nb, sa, sb, base); break;
+    }
+    OMHD_HIP(hipGetLastError());
+  }
+  if (int rc = bl.finish(s)) return rc;
+  if (int rc = br.finish(s)) return rc;
+  a.release(s);
+  bl.release(s);
+  br.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_flux_solver(double *F, const double *VL, const double *VR, double ch, int32_t ix, int32_t jx,
+                                int32_t kx, int32_t dir, int32_t type) {
+  if (!F || !VL || !VR) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", type);
+  if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged f, l, r;
+  if (int rc = f.init(F, 9 * n, false, true, s)) return rc;
+  if (int rc = l.init(VL, 9 * n, true, false, s)) return rc;
+  if (int rc = r.init(VR, 9 * n, true, false, s)) return rc;
+  OMHD_HIP(hipMemsetAsync(f.dev, 0, 9 * n * sizeof(double), s));  // flux_solver.f90:57
+  auto lo = [](int e) { return e < 2 ? e : 2; };
+  auto hi = [](int e) { return e - 1 > 1 ? e - 1 : 1; };
+  int i0 = lo(ix) - 1, ni = hi(ix) - lo(ix) + 1, j0 = lo(jx) - 1, nj = hi(jx) - lo(jx) + 1, k0 = lo(kx) - 1,
+      nk = hi(kx) - lo(kx) + 1;
+  if (dir == 1) { i0 = 0; ni = ix - 1; }
+  if (dir == 2) { j0 = 0; nj = jx - 1; }
+  if (dir == 3) { k0 = 0; nk = kx - 1; }
+  const size_t total = (size_t)(ni > 0 ? ni : 0) * nj * nk;
+  if (total > 0) {
+    if (dir == 1) k_flux<0><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
+    if (dir == 2) k_flux<1><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
+    if (dir == 3) k_flux<2><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
+    OMHD_HIP(hipGetLastError());
+  }
+  if (int rc = f.finish(s)) return rc;
+  f.release(s);
+  l.release(s);
+  r.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_glm_ss2(double *U, double ch, double dt, int32_t ix, int32_t jx, int32_t kx) {
+  if (!U) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  const double cr = 0.2;                    // glm_ss2.f90:18
+  const double f1 = exp(-0.5 * dt * ch / cr);  // :27, host libm
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged a;
+  if (int rc = a.init(U, 9 * n, true, true, s)) return rc;
+  k_scale<<<nblocks(n), TPB, 0, s>>>(a.dev + 8 * n, f1, n);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = a.finish(s)) return rc;
+  a.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_rk(int32_t mode, double *wk, double *wk1, const double *wF, const double *wG, const double *wH,
+                       double dt, double dx, int32_t ix, int32_t jx, int32_t kx) {
+  if (mode < 0 || mode > 3) return set_error(OMHD_ERR_TYPE, "rk mode", mode);
+  if (!wk || !wF || !wG || (mode != 3 && !wk1)) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (ix < 3 || jx < 3 || (wH && kx < 3)) return ok();  // empty interior
+  const double dtx = (mode == 2) ? 0.5 * dt / dx : dt / dx;  // rk.f90:22,55,86,118
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged a, b, f, g, h;
+  if (int rc = a.init(wk, 9 * n, true, mode == 1 || mode == 3, s)) return rc;
+  if (mode != 3)
+    if (int rc = b.init(wk1, 9 * n, true, mode == 0 || mode == 2, s)) return rc;
+  if (int rc = f.init(wF, 9 * n, true, false, s)) return rc;
+  if (int rc = g.init(wG, 9 * n, true, false, s)) return rc;
+  if (wH)
+    if (int rc = h.init(wH, 9 * n, true, false, s)) return rc;
+  const size_t total = (size_t)(ix - 2) * (jx - 2) * (wH ? kx - 2 : 1);
+  k_rk<<<nblocks(total), TPB, 0, s>>>(mode, a.dev, b.dev, f.dev, g.dev, wH ? h.dev : nullptr, dtx, ix, jx, kx);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = a.finish(s)) return rc;
+  if (int rc = b.finish(s)) return rc;
+  a.release(s);
+  b.release(s);
+  f.release(s);
+  g.release(s);
+  h.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_set_dt(const double *U, const double *V, double *vmax, double *dt, double dx, double cfl,
+                           int32_t ix, int32_t jx, int32_t kx, int32_t ndim) {
+  if (!U || !V || !vmax || !dt) return set_error(OMHD_ERR_ARG, "null argument");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged a, b;
+  if (int rc = a.init(U, 9 * n, true, false, s)) return rc;
+  if (int rc = b.init(V, 4 * n, true, false, s)) return rc;
+  unsigned long long *red;
+  OMHD_HIP(hipMallocAsync((void **)&red, 2 * sizeof(unsigned long long), s));
+  OMHD_HIP(hipMemsetAsync(red, 0, 2 * sizeof(unsigned long long), s));
+  const int ni = ix > 2 ? ix - 2 : 1, nj = jx > 2 ? jx - 2 : 1, nk = kx > 2 ? kx - 2 : 1;
+  const size_t total = (size_t)ni * nj * nk;
+  if (ndim == 3)
+    k_speed<3><<<nblocks(total), TPB, 0, s>>>(a.dev, b.dev, ix, jx, kx, red, (int *)(red + 1));
+  else
+    k_speed<2><<<nblocks(total), TPB, 0, s>>>(a.dev, b.dev, ix, jx, kx, red, (int *)(red + 1));
+  OMHD_HIP(hipGetLastError());
+  unsigned long long h[2];
+  OMHD_HIP(hipMemcpyAsync(h, red, sizeof h, hipMemcpyDeviceToHost, s));
+  OMHD_HIP(hipStreamSynchronize(s));
+  (void)hipFreeAsync(red, s);
+  a.release(s);
+  b.release(s);
+  double vm;
+  memcpy(&vm, &h[0], sizeof vm);
+  *vmax = vm;
+  *dt = cfl * dx / vm;  // set_dt.f90:52
+  if ((int)h[1] & 4) return set_error(OMHD_ERR_DT, "non-finite signal speed", 0, 0, 0, vm);
+  if (!(*dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, *dt);  // :55-65
+  return ok();
+}
+
+static int bc_pair(double *dev, int ix, int jx, int kx, int dir, int tlo, int thi, const double *sl, const double *sh,
+                   hipStream_t s) {
+  const int na = (dir == 1) ? jx : ix, nb = (dir == 3) ? jx : kx;
+  const size_t total = (size_t)na * nb;
+  k_bc_face_pair<<<nblocks(total), TPB, 0, s>>>(dev, ix, jx, kx, dir, tlo, thi, sl[0], sl[1], sl[2], sl[3], sl[4],
+                                                 sl[5], sl[6], sl[7], sl[8], sh[0], sh[1], sh[2], sh[3], sh[4], sh[5],
+                                                 sh[6], sh[7], sh[8]);
+  OMHD_HIP(hipGetLastError());
+  return OMHD_OK;
+}
+
+extern "C" int omhd_bc_u(double *U, int32_t ix, int32_t jx, int32_t kx, const int32_t *bc_type, const double *sign) {
+  if (!U || !bc_type) return set_error(OMHD_ERR_ARG, "null argument");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  static const double ones[54] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                  1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  const double *sg = sign ? sign : ones;
+  for (int f = 0; f < 6; f += 2)
+    if (bc_type[f] != bc_type[f + 1] && (bc_type[f] == OMHD_BC_PERIODIC || bc_type[f + 1] == OMHD_BC_PERIODIC))
+      return set_error(OMHD_ERR_ARG, "periodic faces come in pairs", f);
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged a;
+  if (int rc = a.init(U, 9 * n, true, true, s)) return rc;
+  if (ix > 2)
+    if (int rc = bc_pair(a.dev, ix, jx, kx, 1, bc_type[0], bc_type[1], sg, sg + 9, s)) return rc;
+  if (jx > 2)
+    if (int rc = bc_pair(a.dev, ix, jx, kx, 2, bc_type[2], bc_type[3], sg + 18, sg + 27, s)) return rc;
+  if (kx > 2)
+    if (int rc = bc_pair(a.dev, ix, jx, kx, 3, bc_type[4], bc_type[5], sg + 36, sg + 45, s)) return rc;
+  if (int rc = a.finish(s)) return rc;
+  a.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int32_t dir, int32_t bc_type,
+                            const double *sign_lo, const double *sign_hi) {
+  if (!VL || !VR) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
+  static const double ones[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
+  const double *sl = sign_lo ? sign_lo : ones, *sh = sign_hi ? sign_hi : ones;
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged l, r;
+  if (int rc = l.init(VL, 9 * n, true, true, s)) return rc;
+  if (int rc = r.init(VR, 9 * n, true, true, s)) return rc;
+  const int na = (dir == 1) ? jx : ix, nb = (dir == 3) ? jx : kx;
+  const int nl = dir == 1 ? ix : dir == 2 ? jx : kx;
+  if (nl >= 2) {
+    k_face_patch<<<nblocks((size_t)na * nb), TPB, 0, s>>>(l.dev, r.dev, ix, jx, kx, dir, bc_type, sl[0], sl[1], sl[2],
+                                                          sl[3], sl[4], sl[5], sl[6], sl[7], sl[8], sh[0], sh[1],
+                                                          sh[2], sh[3], sh[4], sh[5], sh[6], sh[7], sh[8]);
+    OMHD_HIP(hipGetLastError());
+  }
+  if (int rc = l.finish(s)) return rc;
+  if (int rc = r.finish(s)) return rc;
+  l.release(s);
+  r.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}

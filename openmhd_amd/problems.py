@@ -1,0 +1,109 @@
+"""Initial states of the benchmark configurations, generated on the host in
+float64 and handed to both the GPU step and the CPU reference build (so the
+two runs start from bit-identical states; transcendental functions are only
+evaluated here, never on either side of the comparison).
+
+Arrays are U(ix, jx, 9) in Fortran order with one ghost layer (ix = nx + 2),
+slot order of param.h:6-15.  Conserved variables are formed from primitives
+exactly as 2D_basic/model.f90:46-50 does.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GAMMA = 5.0 / 3.0  # param.h:3
+
+
+def prim_to_cons_fortran(ro, vx, vy, vz, p, bx, by, bz, ps):
+    """(ix,jx) primitive planes -> U(ix,jx,9), model.f90:46-50 order."""
+    ix, jx = ro.shape
+    U = np.empty((ix, jx, 9), order="F")
+    f1 = 1.0 / (GAMMA - 1)
+    v2 = vx * vx + vy * vy + vz * vz
+    B2 = bx * bx + by * by + bz * bz
+    U[:, :, 0] = ro * vx
+    U[:, :, 1] = ro * vy
+    U[:, :, 2] = ro * vz
+    U[:, :, 3] = 0.5 * (ro * v2 + B2) + f1 * p
+    U[:, :, 4] = ro
+    U[:, :, 5] = bx
+    U[:, :, 6] = by
+    U[:, :, 7] = bz
+    U[:, :, 8] = ps
+    return U
+
+
+def orszag_tang(n: int):
+    """Orszag-Tang vortex on [0,2pi]^2 with n x n cells, as 2D_basic/model.f90:
+    rho = gamma^2, p = gamma, v = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0),
+    psi = 0.  Returns (U, dx); ghost cells hold the analytic values (the
+    driver's first bc() overwrites them)."""
+    ix = jx = n + 2
+    pi = 4.0 * np.arctan(1.0)
+    dx = (2 * pi - 0.0) / float(ix - 2)  # model.f90:19
+    x = np.empty(ix)
+    y = np.empty(jx)
+    x[0] = 0.0 - dx / 2
+    for i in range(1, ix):  # model.f90:22-24 (cumulative, as the reference)
+        x[i] = x[i - 1] + dx
+    y[0] = 0.0 - dx / 2
+    for j in range(1, jx):
+        y[j] = y[j - 1] + dx
+    X = np.broadcast_to(x[:, None], (ix, jx))
+    Y = np.broadcast_to(y[None, :], (ix, jx))
+    z = np.zeros((ix, jx))
+    U = prim_to_cons_fortran(np.full((ix, jx), GAMMA ** 2), -np.sin(Y), np.sin(X), z, np.full((ix, jx), GAMMA),
+                             -np.sin(Y), np.sin(2 * X), z, z)
+    return U, dx
+
+
+# -- counter-based Gaussian noise: value depends only on (seed, i, j) ----------
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _M64
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+    return z ^ (z >> np.uint64(31))
+
+
+def counter_normal(seed: int, i: np.ndarray, j: np.ndarray) -> np.ndarray:
+    """Standard normal deviates keyed by (seed, i, j) (Box-Muller on two
+    splitmix64 uniforms); independent of how the grid is tiled."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0xD1B54A32D192ED03)) & _M64
+        c = (i.astype(np.uint64) << np.uint64(32)) | j.astype(np.uint64)
+        a = _splitmix64(c ^ key)
+        b = _splitmix64(a ^ np.uint64(0x632BE59BD9B4E019))
+    u1 = ((a >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+    u2 = ((b >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def kelvin_helmholtz(nx: int, ny: int, seed: int = 1, delta: float = 1.0, lx: float = 40.0, u0: float = 0.5,
+                     beta: float = 2.0, bx: float = 0.2, bz: float = 1.0, noise: float = 1.0e-3):
+    """Kelvin-Helmholtz layer: x in [0, lx*delta) periodic, y in
+    [-ly/2, ly/2] with walls (ly = ny*dx, dx = lx*delta/nx), vx = u0 tanh(y/delta),
+    rho = 1, uniform p = beta |B|^2 / 2, B = (bx, 0, bz), and seeded Gaussian vy
+    noise of amplitude `noise` where |y| < 2 delta.  Returns (U, dx)."""
+    ix, jx = nx + 2, ny + 2
+    dx = lx * delta / nx
+    ly = ny * dx
+    i = np.arange(ix)
+    j = np.arange(jx)
+    x = (i - 0.5) * dx  # cell centres, ghost at i = 0
+    y = -0.5 * ly + (j - 0.5) * dx
+    X = np.broadcast_to(x[:, None], (ix, jx))
+    Y = np.broadcast_to(y[None, :], (ix, jx))
+    I = np.broadcast_to(i[:, None], (ix, jx))
+    J = np.broadcast_to(j[None, :], (ix, jx))
+    p = beta * (bx * bx + bz * bz) / 2.0
+    vx = u0 * np.tanh(Y / delta)
+    vy = np.where(np.abs(Y) < 2.0 * delta, noise * counter_normal(seed, I, J), 0.0)
+    z = np.zeros((ix, jx))
+    one = np.ones((ix, jx))
+    U = prim_to_cons_fortran(one, vx, vy, z, p * one, bx * one, z, bz * one, z)
+    del X
+    return U, dx

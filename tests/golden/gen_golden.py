@@ -1,0 +1,223 @@
+"""Generate the golden fixtures in tests/golden/ from the reference routines.
+
+The reference Fortran (2D_basic, 2D_KH/bc.f90, 3D_basic) is compiled unmodified
+by `make -C oracle ref` into oracle/_ref/*.so; this script calls those
+subroutines on crafted inputs and stores inputs + outputs as .npz (plain
+arrays, no pickles).  Branch labels for the HLLD vectors come from the oracle
+and are used only to choose inputs that cover every branch.
+
+    python tests/golden/gen_golden.py        (needs oracle/_ref, built here)
+"""
+from __future__ import annotations
+
+import hashlib
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+from omhd_testlib import (KH_SIGN, Oracle, Ref, farr, orszag_tang, prim_to_cons, random_prim,  # noqa: E402
+                          ref_run_2d, smooth_state)
+
+OUT = HERE
+
+
+def save(name, **arrs):
+    np.savez_compressed(OUT / f"{name}.npz", **{k: np.asarray(v) for k, v in arrs.items()})
+    print("wrote", name, {k: np.asarray(v).shape for k, v in arrs.items()})
+
+
+def hlld_pool(rng, n):
+    """Interface state pairs spread over the HLLD branches: sub/supersonic
+    flows, weak and strong normal fields, near-HLLC-G switching states."""
+    L = random_prim(rng, (n,))
+    R = random_prim(rng, (n,))
+    kind = rng.integers(0, 6, n)
+    # strong normal flow -> FL / FR
+    s = kind == 0
+    L[s, 0:3] *= 6.0
+    R[s, 0:3] = L[s, 0:3] * rng.uniform(0.8, 1.2, (s.sum(), 3))
+    # tiny normal field -> HLLC-G
+    s = kind == 1
+    L[s, 5:8] *= 1e-4
+    R[s, 5:8] *= 1e-4
+    # strong field, weak pressure -> Alfven close to fast -> HLLC-G
+    s = kind == 2
+    L[s, 5:8] *= 4.0
+    R[s, 5:8] = L[s, 5:8]
+    L[s, 3] *= 0.01
+    R[s, 3] *= 0.01
+    # moderate drift -> L*, R*
+    s = kind == 3
+    off = rng.normal(0, 2.0, (s.sum(), 1))
+    L[s, 0:3] += off
+    R[s, 0:3] += off
+    return L, R
+
+
+def main():
+    orc = Oracle()
+    ref2 = Ref(2)
+    ref3 = Ref(3)
+    rng = np.random.default_rng(20260511)
+
+    # ---- u2v --------------------------------------------------------------
+    ix, jx = 12, 9
+    U = np.asfortranarray(prim_to_cons(random_prim(rng, (ix, jx))))
+    V = farr(ix, jx, 4)
+    ref2.u2v(U, V)
+    save("u2v_2d", U=U, V=V)
+
+    # ---- limiter (2D dir 1,2; 3D dir 3) ----------------------------------
+    out = {}
+    for typ in range(5):
+        for d in (1, 2):
+            wk = np.asfortranarray(rng.normal(size=(11, 10)))
+            wk[3, 3], wk[4, 3], wk[5, 3] = -0.0, 0.0, -0.0  # signed-zero gradients
+            wk[6, 5] = wk[7, 5]  # gB == 0
+            wk[2, 6] = wk[2, 7] = wk[2, 8]
+            wL = np.full((11, 10), 7.0, order="F")
+            wR = np.full((11, 10), -7.0, order="F")
+            ref2.limiter(wk, wL, wR, d, typ)
+            out[f"wk_{typ}_{d}"] = wk
+            out[f"wL_{typ}_{d}"] = wL
+            out[f"wR_{typ}_{d}"] = wR
+        wk = np.asfortranarray(rng.normal(size=(6, 5, 9)))
+        wk[2, 2, 3] = -0.0
+        wL = np.full((6, 5, 9), 7.0, order="F")
+        wR = np.full((6, 5, 9), -7.0, order="F")
+        ref3.limiter(wk, wL, wR, 3, typ)
+        out[f"wk_{typ}_3"] = wk
+        out[f"wL_{typ}_3"] = wL
+        out[f"wR_{typ}_3"] = wR
+    save("limiter", **out)
+
+    # ---- HLLD: every branch x dir ----------------------------------------
+    out = {}
+    for d in (1, 2, 3):
+        L, R = hlld_pool(rng, 20000)
+        br = np.array([orc.hlld(L[i], R[i], 1.3, d - 1)[1] for i in range(len(L))])
+        pick = []
+        for b in range(1, 9):
+            idx = np.flatnonzero(br == b)
+            assert len(idx) >= 4, f"branch {b} dir {d} not covered ({len(idx)})"
+            pick.extend(idx[:8].tolist())
+        pick = np.array(pick)
+        n = len(pick)
+        if d < 3:  # lay the pairs out along the interface direction of a 2D grid
+            shp = (n + 2, 3) if d == 1 else (3, n + 2)
+            VL = farr(*shp, 9)
+            VR = farr(*shp, 9)
+            VL[...] = 1.0
+            VR[...] = 1.0
+            if d == 1:
+                VL[1:n + 1, 1, :] = L[pick]
+                VR[1:n + 1, 1, :] = R[pick]
+            else:
+                VL[1, 1:n + 1, :] = L[pick]
+                VR[1, 1:n + 1, :] = R[pick]
+            F = farr(*shp, 9)
+            ref2.flux_solver(F, VL, VR, 1.3, d)
+        else:
+            shp = (3, 3, n + 2)
+            VL = farr(*shp, 9)
+            VR = farr(*shp, 9)
+            VL[...] = 1.0
+            VR[...] = 1.0
+            VL[1, 1, 1:n + 1, :] = L[pick]
+            VR[1, 1, 1:n + 1, :] = R[pick]
+            F = farr(*shp, 9)
+            ref3.flux_solver(F, VL, VR, 1.3, d)
+        out[f"VL_{d}"], out[f"VR_{d}"], out[f"F_{d}"] = VL, VR, F
+        out[f"branch_{d}"] = br[pick]
+    out["ch"] = np.array(1.3)
+    save("hlld", **out)
+
+    # ---- glm_ss2, rk (2D/3D), set_dt (2D/3D), bc ---------------------------
+    out = {}
+    U = np.asfortranarray(prim_to_cons(random_prim(rng, (10, 8))))
+    U2 = U.copy(order="F")
+    ref2.glm_ss2(U2, 1.37, 0.0123)
+    out["glm_in"], out["glm_out"] = U, U2
+    F = np.asfortranarray(rng.normal(size=(10, 8, 9)))
+    G = np.asfortranarray(rng.normal(size=(10, 8, 9)))
+    for mode in range(4):
+        A = U.copy(order="F")
+        B = np.asfortranarray(rng.normal(size=(10, 8, 9)))
+        out[f"rk2d_{mode}_in_wk"], out[f"rk2d_{mode}_in_wk1"] = A.copy(order="F"), B.copy(order="F")
+        ref2.rk(mode, A, B, F, G, None, 0.0123, 0.0371)
+        out[f"rk2d_{mode}_wk"], out[f"rk2d_{mode}_wk1"] = A, B
+    out["rk2d_F"], out["rk2d_G"] = F, G
+    U3 = np.asfortranarray(prim_to_cons(random_prim(rng, (6, 5, 7))))
+    F3, G3, H3 = (np.asfortranarray(rng.normal(size=(6, 5, 7, 9))) for _ in range(3))
+    for mode in range(4):
+        A = U3.copy(order="F")
+        B = np.asfortranarray(rng.normal(size=(6, 5, 7, 9)))
+        out[f"rk3d_{mode}_in_wk"], out[f"rk3d_{mode}_in_wk1"] = A.copy(order="F"), B.copy(order="F")
+        ref3.rk(mode, A, B, F3, G3, H3, 0.0123, 0.0371)
+        out[f"rk3d_{mode}_wk"], out[f"rk3d_{mode}_wk1"] = A, B
+    out["rk3d_F"], out["rk3d_G"], out["rk3d_H"] = F3, G3, H3
+    V = farr(10, 8, 4)
+    ref2.u2v(U, V)
+    vmax, dt = ref2.set_dt(U, V, 0.0371, 0.4)
+    out["setdt2d_U"], out["setdt2d_V"], out["setdt2d_out"] = U, V, np.array([vmax, dt])
+    V3 = farr(6, 5, 7, 4)
+    ref3.u2v(U3, V3)
+    vmax, dt = ref3.set_dt(U3, V3, 0.0371, 0.4)
+    out["setdt3d_U"], out["setdt3d_V"], out["setdt3d_out"] = U3, V3, np.array([vmax, dt])
+    A = np.asfortranarray(rng.normal(size=(7, 6, 9)))
+    out["bc_in"] = A.copy(order="F")
+    ref2.bc(A)
+    out["bc_out"] = A
+    A = np.asfortranarray(rng.normal(size=(7, 6, 9)))
+    out["bckh_in"] = A.copy(order="F")
+    ref2.bc_for_U(A)
+    out["bckh_out"] = A
+    VL = np.asfortranarray(rng.normal(size=(7, 6, 9)))
+    VR = np.asfortranarray(rng.normal(size=(7, 6, 9)))
+    out["bcF_inL"], out["bcF_inR"] = VL.copy(order="F"), VR.copy(order="F")
+    ref2.bc_for_F(VL, VR)
+    out["bcF_L"], out["bcF_R"] = VL.copy(order="F"), VR.copy(order="F")
+    ref2.bc_for_G(VL, VR)
+    out["bcG_L"], out["bcG_R"] = VL, VR
+    A = np.asfortranarray(rng.normal(size=(6, 5, 7, 9)))
+    out["bc3d_in"] = A.copy(order="F")
+    ref3.bc(A)
+    out["bc3d_out"] = A
+    save("small", **out)
+
+    # ---- whole runs: OT 200^2 MC 1000 steps, KH walls 128^2 minmod 100 ------
+    U, dx = orszag_tang(200)
+    ref2.bc(U)
+    U0sha = hashlib.sha256(U.tobytes(order="F")).hexdigest()
+    dts, chs = ref_run_2d(ref2, U, 1000, dx, 0.4, 2)
+    interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
+    save("ot200_mc_1000", dt=dts, ch=chs, dx=np.array(dx), u0_sha256=np.array(U0sha),
+         final_sha256=np.array(hashlib.sha256(interior.tobytes()).hexdigest()),
+         field_sum=interior.sum(axis=(1, 2)), field_absmax=np.abs(interior).max(axis=(1, 2)),
+         sample=interior[:, ::7, ::7])
+
+    sys.path.insert(0, str(HERE.parents[1]))
+    from openmhd_amd.problems import kelvin_helmholtz  # noqa: E402
+
+    U, dx = kelvin_helmholtz(128, 128, seed=7)
+    ref2.bc_for_U(U)
+    U0 = U.copy(order="F")
+    dts, chs = ref_run_2d(ref2, U, 100, dx, 0.4, 1, bc="kh")
+    save("kh128_minmod_100", U0=U0, dt=dts, ch=chs, dx=np.array(dx), final=U)
+
+    # ---- small 2D periodic run with every limiter (10 steps) ---------------
+    out = {}
+    for lm in range(5):
+        U = smooth_state(np.random.default_rng(lm), 20, 18)
+        ref2.bc(U)
+        out[f"U0_{lm}"] = U.copy(order="F")
+        dts, chs = ref_run_2d(ref2, U, 10, 0.05, 0.4, lm)
+        out[f"U_{lm}"], out[f"dt_{lm}"] = U, dts
+    save("limiters_run", **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+"""The resident fused step (Solver2D / omhd_step) against the oracle and the
+reference's golden runs.  Bitwise: identical dt sequence and state."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from omhd_testlib import make_problem, orszag_tang, smooth_state
+from test_oracle_golden import bits_equal, load
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(nx, ny, dx, lm=2, tt=0, kh=False, rows=0):
+    from openmhd_amd import Solver2D
+
+    return Solver2D(nx, ny, dx, cfl=0.4, lm_type=lm, time_type=tt,
+                    bc=("periodic", "wall" if kh else "periodic"), rows_per_wave=rows)
+
+
+def run_both(oracle, U, nsteps, dx, lm=2, tt=0, kh=False, rows=0):
+    ix, jx = U.shape[:2]
+    p = make_problem(ix, jx, dx=dx, cfl=0.4, lm_type=lm, time_type=tt,
+                     bc=("periodic", "wall" if kh else "periodic", "periodic"))
+    oracle.bc_u(U, p)
+    s = solver(ix - 2, jx - 2, dx, lm, tt, kh, rows)
+    s.upload(U)
+    dts, chs = s.step(nsteps, trace=True)
+    Ug = s.download()
+    s.close()
+    Uo = U.copy(order="F")
+    rc, dto, cho = oracle.run(p, Uo, nsteps)
+    assert rc == 0
+    return dts, chs, Ug, dto, cho, Uo
+
+
+@pytest.mark.parametrize("lm", range(5))
+@pytest.mark.parametrize("tt", (0, 1))
+def test_every_limiter_and_time_type(oracle, lm, tt):
+    rng = np.random.default_rng(lm + 10 * tt)
+    U = smooth_state(rng, 75, 47)  # partial strips and chunks
+    dts, chs, Ug, dto, cho, Uo = run_both(oracle, U, 6, 0.05, lm, tt)
+    assert bits_equal(dts, dto) and bits_equal(chs, cho)
+    assert bits_equal(Ug, Uo)
+
+
+@pytest.mark.parametrize("rows", (1, 3, 16, 64))
+@pytest.mark.parametrize("kh", (False, True))
+def test_chunking_and_walls(oracle, rows, kh):
+    rng = np.random.default_rng(rows)
+    U = smooth_state(rng, 130, 70)
+    dts, chs, Ug, dto, cho, Uo = run_both(oracle, U, 8, 0.05, 1, 0, kh, rows)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
+
+
+def test_golden_limiter_runs():
+    d = load("limiters_run")
+    for lm in range(5):
+        U = np.asfortranarray(d[f"U0_{lm}"]).copy(order="F")
+        s = solver(18, 16, 0.05, lm)
+        s.upload(U)
+        dts, _ = s.step(10, trace=True)
+        Ug = s.download()
+        assert bits_equal(dts, d[f"dt_{lm}"]), lm
+        assert bits_equal(Ug[1:-1, 1:-1], d[f"U_{lm}"][1:-1, 1:-1]), lm
+        assert bits_equal(Ug, d[f"U_{lm}"]), lm
+
+
+def test_golden_kh_walls():
+    d = load("kh128_minmod_100")
+    U = np.asfortranarray(d["U0"]).copy(order="F")
+    s = solver(128, 128, float(d["dx"]), 1, 0, kh=True)
+    s.upload(U)
+    dts, chs = s.step(100, trace=True)
+    Ug = s.download()
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
+    assert bits_equal(Ug, d["final"])
+
+
+def test_orszag_tang_1000_steps():
+    """Config C1 (OT 200^2, MC, 1000 steps): dt sequence and final state
+    bitwise equal to the reference CPU build."""
+    d = load("ot200_mc_1000")
+    U, dx = orszag_tang(200)
+    from openmhd_amd import api
+
+    api.bc(U, 202, 202)
+    assert hashlib.sha256(U.tobytes(order="F")).hexdigest() == str(d["u0_sha256"])
+    s = solver(200, 200, dx, 2)
+    s.upload(U)
+    dts, chs = s.step(1000, trace=True)
+    Ug = s.download()
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
+    interior = np.ascontiguousarray(Ug[1:-1, 1:-1, :].transpose(2, 1, 0))
+    assert bits_equal(interior[:, ::7, ::7], d["sample"])
+    assert hashlib.sha256(interior.tobytes()).hexdigest() == str(d["final_sha256"])
+
+
+def test_kh_4096_two_steps_vs_oracle(oracle):
+    """Config C2's grid (4096^2 KH, minmod, walls): two full steps on the GPU
+    equal the oracle's bitwise at full size."""
+    from openmhd_amd.problems import kelvin_helmholtz
+
+    U, dx = kelvin_helmholtz(4096, 4096, seed=1)
+    dts, chs, Ug, dto, cho, Uo = run_both(oracle, U, 2, dx, 1, 0, kh=True)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
+
+
+def test_negative_pressure_is_reported():
+    from openmhd_amd import OMHDError
+
+    rng = np.random.default_rng(0)
+    U = smooth_state(rng, 34, 34)
+    U[10, 12, 3] = 0.0  # E = 0 -> p < 0 at Fortran cell (11, 13)
+    s = solver(32, 32, 0.05)
+    with pytest.raises(OMHDError) as e:
+        s.upload(U)
+    assert e.value.code == 1 and e.value.cell[:2] == (11, 13)

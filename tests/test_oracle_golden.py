@@ -1,0 +1,132 @@
+"""The oracle (C restatement) against the golden vectors produced by the
+reference's own routines (tests/golden/gen_golden.py).  Bitwise."""
+import hashlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from omhd_testlib import KH_SIGN, farr, make_problem, orszag_tang
+
+G = Path(__file__).resolve().parent / "golden"
+
+
+def load(name):
+    return np.load(G / f"{name}.npz", allow_pickle=False)
+
+
+def bits_equal(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def test_u2v(oracle):
+    d = load("u2v_2d")
+    V = farr(*d["V"].shape)
+    rc, _ = oracle.u2v(np.asfortranarray(d["U"]), V)
+    assert rc == 0 and bits_equal(V, d["V"])
+
+
+@pytest.mark.parametrize("typ", range(5))
+@pytest.mark.parametrize("dir", (1, 2, 3))
+def test_limiter(oracle, typ, dir):
+    d = load("limiter")
+    wk = np.asfortranarray(d[f"wk_{typ}_{dir}"])
+    wL = np.full(wk.shape, 7.0, order="F")
+    wR = np.full(wk.shape, -7.0, order="F")
+    assert oracle.limiter(wk, wL, wR, dir, typ) == 0
+    assert bits_equal(wL, d[f"wL_{typ}_{dir}"])
+    assert bits_equal(wR, d[f"wR_{typ}_{dir}"])
+
+
+@pytest.mark.parametrize("dir", (1, 2, 3))
+def test_hlld_every_branch(oracle, dir):
+    d = load("hlld")
+    VL, VR = np.asfortranarray(d[f"VL_{dir}"]), np.asfortranarray(d[f"VR_{dir}"])
+    F = farr(*VL.shape)
+    br = np.zeros(VL.shape[:-1], dtype=np.int32, order="F")
+    assert oracle.flux_solver(F, VL, VR, float(d["ch"]), dir, 3, branches=br) == 0
+    assert bits_equal(F, d[f"F_{dir}"])
+    # the fixture covers all eight outcomes of flux_solver.f90:413-572
+    assert set(np.unique(d[f"branch_{dir}"])) == set(range(1, 9))
+
+
+def test_glm_rk_setdt_bc(oracle):
+    d = load("small")
+    U = np.asfortranarray(d["glm_in"]).copy(order="F")
+    oracle.glm_ss2(U, 1.37, 0.0123)
+    assert bits_equal(U, d["glm_out"])
+    for dim, H in (("2d", None), ("3d", "rk3d_H")):
+        F, Gf = d[f"rk{dim}_F"], d[f"rk{dim}_G"]
+        Hh = np.asfortranarray(d[H]) if H else None
+        for mode in range(4):
+            A = np.asfortranarray(d[f"rk{dim}_{mode}_in_wk"]).copy(order="F")
+            B = np.asfortranarray(d[f"rk{dim}_{mode}_in_wk1"]).copy(order="F")
+            oracle.rk(mode, A, B, np.asfortranarray(F), np.asfortranarray(Gf), Hh, 0.0123, 0.0371)
+            assert bits_equal(A, d[f"rk{dim}_{mode}_wk"]), (dim, mode)
+            assert bits_equal(B, d[f"rk{dim}_{mode}_wk1"]), (dim, mode)
+    rc, vmax, dt = oracle.set_dt(np.asfortranarray(d["setdt2d_U"]), np.asfortranarray(d["setdt2d_V"]), 0.0371, 0.4, 2)
+    assert rc == 0 and bits_equal([vmax, dt], d["setdt2d_out"])
+    rc, vmax, dt = oracle.set_dt(np.asfortranarray(d["setdt3d_U"]), np.asfortranarray(d["setdt3d_V"]), 0.0371, 0.4, 3)
+    assert rc == 0 and bits_equal([vmax, dt], d["setdt3d_out"])
+    A = np.asfortranarray(d["bc_in"]).copy(order="F")
+    oracle.bc_u(A, make_problem(7, 6))
+    assert bits_equal(A, d["bc_out"])
+    A = np.asfortranarray(d["bc3d_in"]).copy(order="F")
+    oracle.bc_u(A, make_problem(6, 5, 7))
+    assert bits_equal(A, d["bc3d_out"])
+    A = np.asfortranarray(d["bckh_in"]).copy(order="F")
+    oracle.bc_u(A, make_problem(7, 6, bc=("periodic", "wall", "periodic")))
+    assert bits_equal(A, d["bckh_out"])
+
+
+def test_face_patch(oracle):
+    d = load("small")
+    import ctypes as C
+
+    from omhd_testlib import ptr
+
+    VL = np.asfortranarray(d["bcF_inL"]).copy(order="F")
+    VR = np.asfortranarray(d["bcF_inR"]).copy(order="F")
+    one = np.ones(9)
+    oracle.L.or_bc_face(ptr(VL), ptr(VR), 7, 6, 1, 1, 0, ptr(one), ptr(one))
+    assert bits_equal(VL, d["bcF_L"]) and bits_equal(VR, d["bcF_R"])
+    s = np.ascontiguousarray(KH_SIGN)
+    oracle.L.or_bc_face(ptr(VL), ptr(VR), 7, 6, 1, 2, 1, ptr(s), ptr(s))
+    assert bits_equal(VL, d["bcG_L"]) and bits_equal(VR, d["bcG_R"])
+    del C
+
+
+@pytest.mark.parametrize("lm", range(5))
+def test_every_limiter_run(oracle, lm):
+    d = load("limiters_run")
+    U = np.asfortranarray(d[f"U0_{lm}"]).copy(order="F")
+    rc, dts, _ = oracle.run(make_problem(20, 18, dx=0.05, cfl=0.4, lm_type=lm), U, 10)
+    assert rc == 0
+    assert bits_equal(dts, d[f"dt_{lm}"]) and bits_equal(U, d[f"U_{lm}"])
+
+
+def test_kh_walls_run(oracle):
+    d = load("kh128_minmod_100")
+    U = np.asfortranarray(d["U0"]).copy(order="F")
+    p = make_problem(130, 130, dx=float(d["dx"]), cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"))
+    rc, dts, chs = oracle.run(p, U, 100)
+    assert rc == 0
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
+    assert bits_equal(U, d["final"])
+
+
+def test_orszag_tang_1000_steps(oracle):
+    """Config C1: OT 200^2, MC, HLLD, GLM, TVD-RK2, CFL 0.4, 1000 steps: the
+    dt sequence and the final state are bitwise those of the reference."""
+    d = load("ot200_mc_1000")
+    U, dx = orszag_tang(200)
+    p = make_problem(202, 202, dx=dx, cfl=0.4, lm_type=2)
+    oracle.bc_u(U, p)
+    assert hashlib.sha256(U.tobytes(order="F")).hexdigest() == str(d["u0_sha256"])
+    rc, dts, chs = oracle.run(p, U, 1000)
+    assert rc == 0
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
+    interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
+    assert hashlib.sha256(interior.tobytes()).hexdigest() == str(d["final_sha256"])
