@@ -45,18 +45,25 @@ def cpu_baseline(n: int, sample_steps: int):
     """The reference CPU build (oracle/_ref, the project's Fortran driver over the
     reference's own routines, OpenMP) timed on this host on a bounded sample of
     the same workload; falls back to the C restatement if _ref is absent."""
+    import resource
+
     threads = min(16, os.cpu_count() or 1)
     drv = REPO / "oracle" / "_ref" / "main2d_ref_omp"
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+
+    def big_stack():  # set_dt/glm_ss2 keep ix*jx automatic arrays on the stack
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
     if drv.exists():
         cmd = [str(drv), "--kh", str(n), str(sample_steps + 1), "--time"]
         log("cpu baseline:", " ".join(cmd))
-        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900, preexec_fn=big_stack)
         m = re.search(r"zone_updates_per_s\s+([0-9.eE+]+)", out.stdout)
         if out.returncode == 0 and m:
             return {"value": float(m.group(1)), "unit": "zone-updates/s", "cores": threads, "kind": "reference",
-                    "sample": f"KH {n}x{n} minmod, steps 2..{sample_steps + 1} of a {sample_steps + 1}-step run, "
-                              f"reference routines built with flang -O2 -fopenmp, {threads} threads"}
+                    "sample": f"KH {n}x{n} minmod, steps 2..{sample_steps + 1} of a {sample_steps + 1}-step run "
+                              f"(driver's own KH layer init), the reference's routines compiled unmodified with "
+                              f"flang -O2 -fopenmp, {threads} OpenMP threads"}
         log("reference driver failed:", out.stdout[-500:], out.stderr[-500:])
     # C restatement (port)
     sys.path.insert(0, str(REPO / "tests"))
@@ -84,12 +91,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
+    ap.add_argument("--nt", type=int, default=0, help="1: non-temporal stores of the stage outputs")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py: this build runs on one GPU only (the grid is one domain on one MI355X); "
                  "multi-GPU sharding is a design note, not built -- run with --gpus 1")
+
+    # the CPU baseline runs first, as a child process started before this
+    # process initialises the GPU
+    cpu = None
+    if not a.no_cpu and not a.calibrate_copy:
+        try:
+            cpu = cpu_baseline(a.n, a.cpu_sample_steps)
+        except Exception as e:  # the baseline is reported, never the measured value
+            log("cpu baseline failed:", repr(e))
 
     import torch
 
@@ -98,10 +117,20 @@ def main():
     from openmhd_amd.problems import kelvin_helmholtz
 
     torch.cuda.set_device(0)
+    if a.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "launches": 5}))
+        return
     n = a.n
     log(f"building KH {n}x{n} initial state on the host")
     U, dx = kelvin_helmholtz(n, n, seed=1)
-    s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows)
+    s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows,
+                 store_hint=a.nt)
     s.upload(U)
     del U
     log(f"warmup {a.warmup} steps")
@@ -123,12 +152,6 @@ def main():
     achieved = BYTES_STAGE[dom + 1] * cells / (st[dom] * 1e-3) / 1e9
     traffic = parse_traffic()
     log(f"{ms_step:.3f} ms/step, {zups / 1e9:.3f} G zone-updates/s; stage1 {st[0]:.3f} ms, stage2 {st[1]:.3f} ms")
-    cpu = None
-    if not a.no_cpu:
-        try:
-            cpu = cpu_baseline(n, a.cpu_sample_steps)
-        except Exception as e:  # the baseline is reported, never the measured value
-            log("cpu baseline failed:", repr(e))
     out = {
         "metric": "FP64 zone-updates/s & %HBM peak; batched and adjoint time vs plain GPU step",
         "value": zups,

@@ -114,7 +114,8 @@ typedef struct omhd_config {
   int32_t bc_type[6];   /* OMHD_BC_* per face: x-, x+, y-, y+, z-, z+       */
   double bc_sign[54];   /* mirror sign per face and slot (6 x 9)            */
   int32_t rows_per_wave;/* 0: default; sweep chunk height (tuning)          */
-  int32_t reserved[7];
+  int32_t store_hint;   /* 0: plain stores, 1: non-temporal stores (tuning) */
+  int32_t reserved[6];
 } omhd_config;
 
 typedef struct omhd_handle omhd_handle;

@@ -41,7 +41,7 @@ class Config(C.Structure):
         ("dx", C.c_double), ("cfl", C.c_double),
         ("lm_type", I32), ("flux_type", I32), ("time_type", I32),
         ("bc_type", I32 * 6), ("bc_sign", C.c_double * 54),
-        ("rows_per_wave", I32), ("reserved", I32 * 7),
+        ("rows_per_wave", I32), ("store_hint", I32), ("reserved", I32 * 6),
     ]
 
 

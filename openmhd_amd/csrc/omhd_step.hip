@@ -75,8 +75,18 @@ __device__ __forceinline__ void record_error(int *err, int code, int i, int j) {
   }
 }
 
-template <int LIM, int STAGE, int TT>
-__global__ __launch_bounds__(64) void k_sweep2d(const Sweep2D p) {
+// Per wavefront LDS: a ring of three primitive rows (rn-2, rn-1, rn) and the
+// carried left face state of the y-interface above row rn-2.  Keeping these
+// out of VGPRs leaves ~18 live doubles across each inlined HLLD call, so two
+// wavefronts fit per SIMD.
+struct SweepLds {
+  double ring[3][NV][66];  // [slot][var][lane + 1]; entries 0 and 65 repeat lanes 0 and 63
+  double wly[NV][64];
+};
+
+template <int LIM, int STAGE, int TT, int NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep2d(const Sweep2D p) {
+  __shared__ SweepLds lds;
   const int lane = threadIdx.x;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
@@ -93,7 +103,7 @@ __global__ __launch_bounds__(64) void k_sweep2d(const Sweep2D p) {
   const double glm_us = (STAGE == 1) ? p.glm_f : 1.0;
   const long long plane = p.plane;
 
-  double A[NV], B[NV], W[NV], wLyA[NV], Gl[NV], nxt[NV];
+  double Gl[NV], nxt[NV];
   double vloc = 0.0;
 
   auto row_addr = [&](int rn, unsigned &mask) -> long long {
@@ -113,36 +123,68 @@ __global__ __launch_bounds__(64) void k_sweep2d(const Sweep2D p) {
 
   for (int k = 0; k < hc + 4; ++k) {
     const int rn = j0 - 2 + k;
-    double u[NV];
+    const int sa = (k + 1) % 3, sb = (k + 2) % 3, sc = k % 3;  // rows rn-2, rn-1, rn
+    {
+      double u[NV], W[NV];
 #pragma unroll
-    for (int m = 0; m < NV; m++) u[m] = flip(nxt[m], mask_nxt, m);
-    u[8] = glm_us * u[8];  // glm_ss2 before the sweep (main.f90:87)
-    if (k + 1 < hc + 4) {  // prefetch the next row
-      const long long o = row_addr(rn + 1, mask_nxt);
+      for (int m = 0; m < NV; m++) u[m] = flip(nxt[m], mask_nxt, m);
+      u[8] = glm_us * u[8];  // glm_ss2 before the sweep (main.f90:87)
+      if (k + 1 < hc + 4) {  // prefetch the next row
+        const long long o = row_addr(rn + 1, mask_nxt);
 #pragma unroll
-      for (int m = 0; m < NV; m++) nxt[m] = p.us[m * plane + o];
+        for (int m = 0; m < NV; m++) nxt[m] = p.us[m * plane + o];
+      }
+      cons_to_prim(u, W);  // u2v
+      if (!(W[3] > 0.0) || !(u[4] > 0.0)) {
+        int rm;
+        unsigned dummy;
+        map_axis(rn, p.ny, p.bcy, p.sy_lo, p.sy_hi, rm, dummy);
+        record_error(p.err, (W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, cm + 2, rm + 2);
+      }
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        lds.ring[sc][m][lane + 1] = W[m];
+        if (lane == 0) lds.ring[sc][m][0] = W[m];
+        if (lane == 63) lds.ring[sc][m][65] = W[m];
+      }
     }
-    cons_to_prim(u, W);  // u2v
-    if (!(W[3] > 0.0) || !(u[4] > 0.0)) {
-      int rm;
-      unsigned dummy;
-      map_axis(rn, p.ny, p.bcy, p.sy_lo, p.sy_hi, rm, dummy);
-      record_error(p.err, (W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, cm + 2, rm + 2);
+    __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
+
+    // x-sweep of row r = rn-2: tq = (F(i-1) - F(i)) + G(j-1)   (rk.f90:30-31 order)
+    double tq[NV];
+    if (k >= 4) {
+      double Lx[NV], Rx[NV], F[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        double wRx;
+        limit<LIM>(lds.ring[sa][m][lane], lds.ring[sa][m][lane + 1], lds.ring[sa][m][lane + 2], wRx, Lx[m]);
+        Rx[m] = from_next(wRx);
+      }
+      if (xlo_face) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) Lx[m] = flip(Rx[m], p.sx_lo, m);
+      }
+      if (xhi_face) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) Rx[m] = flip(Lx[m], p.sx_hi, m);
+      }
+      hlld<0>(Lx, Rx, p.ch, p.ch2, F);
+#pragma unroll
+      for (int m = 0; m < NV; m++) tq[m] = from_prev(F[m]) - F[m] + Gl[m];
     }
 
     if (k >= 2) {
-      // y-slopes of row rn-1 (window B) from rows rn-2, rn-1, rn
-      double wRyB[NV], wLyB[NV];
+      // y-slopes of row rn-1 from rows rn-2, rn-1, rn
+      double Lf[NV], Rf[NV];
 #pragma unroll
-      for (int m = 0; m < NV; m++) limit<LIM>(A[m], B[m], W[m], wRyB[m], wLyB[m]);
+      for (int m = 0; m < NV; m++) {
+        double wLyB;
+        limit<LIM>(lds.ring[sa][m][lane + 1], lds.ring[sb][m][lane + 1], lds.ring[sc][m][lane + 1], Rf[m], wLyB);
+        Lf[m] = lds.wly[m][lane];  // left state of the interface above row rn-2
+        lds.wly[m][lane] = wLyB;
+      }
       if (k >= 3) {
         // y-interface between rows rn-2 and rn-1
-        double Lf[NV], Rf[NV], G[NV];
-#pragma unroll
-        for (int m = 0; m < NV; m++) {
-          Lf[m] = wLyA[m];
-          Rf[m] = wRyB[m];
-        }
         if (p.bcy == OMHD_BC_MIRROR) {
           if (rn - 1 == 0) {  // south wall: bc_for_G, 2D_KH/bc.f90:62-70
 #pragma unroll
@@ -153,72 +195,47 @@ __global__ __launch_bounds__(64) void k_sweep2d(const Sweep2D p) {
             for (int m = 0; m < NV; m++) Rf[m] = flip(Lf[m], p.sy_hi, m);
           }
         }
+        double G[NV];
         hlld<1>(Lf, Rf, p.ch, p.ch2, G);
 
-        if (k >= 4) {
-          // x-sweep of row r = rn-2 (window A)
+        if (k >= 4 && writer) {
           const int r = rn - 2;
-          double wRx[NV], wLx[NV], Lx[NV], Rx[NV], F[NV];
+          const long long o = (long long)r * p.pitch + c;
+          double o9[NV];
+          if constexpr (STAGE == 1) {
 #pragma unroll
-          for (int m = 0; m < NV; m++) limit<LIM>(from_prev(A[m]), A[m], from_next(A[m]), wRx[m], wLx[m]);
-#pragma unroll
-          for (int m = 0; m < NV; m++) {
-            Lx[m] = wLx[m];
-            Rx[m] = from_next(wRx[m]);
-          }
-          if (xlo_face) {
-#pragma unroll
-            for (int m = 0; m < NV; m++) Lx[m] = flip(Rx[m], p.sx_lo, m);
-          }
-          if (xhi_face) {
-#pragma unroll
-            for (int m = 0; m < NV; m++) Rx[m] = flip(Lx[m], p.sx_hi, m);
-          }
-          hlld<0>(Lx, Rx, p.ch, p.ch2, F);
-          double d[NV];
-#pragma unroll
-          for (int m = 0; m < NV; m++) d[m] = from_prev(F[m]) - F[m] + Gl[m] - G[m];  // rk.f90:30-31
-
-          if (writer) {
-            const long long o = (long long)r * p.pitch + c;
-            double o9[NV];
-            if constexpr (STAGE == 1) {
-#pragma unroll
-              for (int m = 0; m < NV; m++) {
-                double uu = p.us[m * plane + o];
-                if (m == 8) uu = p.glm_f * uu;
-                o9[m] = uu + p.dtx * d[m];  // rk_tvd21 / rk_std21
-              }
-            } else {
-#pragma unroll
-              for (int m = 0; m < NV; m++) {
-                double a0 = p.u0[m * plane + o];
-                if (m == 8) a0 = p.glm_f * a0;
-                if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + p.dtx * d[m]);  // rk_tvd22
-                else o9[m] = a0 + p.dtx * d[m];                                                   // rk_std22
-              }
-              o9[8] = p.glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
+            for (int m = 0; m < NV; m++) {
+              double uu = p.us[m * plane + o];
+              if (m == 8) uu = p.glm_f * uu;
+              o9[m] = uu + p.dtx * (tq[m] - G[m]);  // rk_tvd21 / rk_std21
             }
+          } else {
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+              double a0 = p.u0[m * plane + o];
+              if (m == 8) a0 = p.glm_f * a0;
+              if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + p.dtx * (tq[m] - G[m]));  // rk_tvd22
+              else o9[m] = a0 + p.dtx * (tq[m] - G[m]);                                                   // rk_std22
+            }
+            o9[8] = p.glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
+          }
+          if constexpr (NT) {  // streaming store: the next launch reads it, this one does not
+#pragma unroll
+            for (int m = 0; m < NV; m++) __builtin_nontemporal_store(o9[m], &p.out[m * plane + o]);
+          } else {
 #pragma unroll
             for (int m = 0; m < NV; m++) p.out[m * plane + o] = o9[m];
-            if constexpr (STAGE == 2) {  // next step's set_dt on U^{n+1}
-              double w2[NV];
-              cons_to_prim(o9, w2);
-              const double vt = fastest_speed<2>(o9, w2);
-              vloc = (vt > vloc || !(vt == vt)) ? vt : vloc;
-            }
+          }
+          if constexpr (STAGE == 2) {  // next step's set_dt on U^{n+1}
+            double w2[NV];
+            cons_to_prim(o9, w2);
+            const double vt = fastest_speed<2>(o9, w2);
+            vloc = (vt > vloc || !(vt == vt)) ? vt : vloc;
           }
         }
 #pragma unroll
         for (int m = 0; m < NV; m++) Gl[m] = G[m];
       }
-#pragma unroll
-      for (int m = 0; m < NV; m++) wLyA[m] = wLyB[m];
-    }
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-      A[m] = B[m];
-      B[m] = W[m];
     }
   }
   if constexpr (STAGE == 2) {
@@ -296,22 +313,27 @@ __global__ void k_zero_red(unsigned long long *red) {
 
 typedef void (*sweep_fn)(const Sweep2D);
 
-template <int LIM>
+template <int LIM, int NT>
 void pick(int stage, int tt, sweep_fn &f) {
-  if (stage == 1) f = tt == 0 ? k_sweep2d<LIM, 1, 0> : k_sweep2d<LIM, 1, 1>;
-  else f = tt == 0 ? k_sweep2d<LIM, 2, 0> : k_sweep2d<LIM, 2, 1>;
+  if (stage == 1) f = tt == 0 ? k_sweep2d<LIM, 1, 0, NT> : k_sweep2d<LIM, 1, 1, NT>;
+  else f = tt == 0 ? k_sweep2d<LIM, 2, 0, NT> : k_sweep2d<LIM, 2, 1, NT>;
 }
 
-sweep_fn sweep_kernel(int lim, int stage, int tt) {
+template <int NT>
+sweep_fn sweep_kernel_nt(int lim, int stage, int tt) {
   sweep_fn f = nullptr;
   switch (lim) {
-    case 0: pick<0>(stage, tt, f); break;
-    case 1: pick<1>(stage, tt, f); break;
-    case 2: pick<2>(stage, tt, f); break;
-    case 3: pick<3>(stage, tt, f); break;
-    default: pick<4>(stage, tt, f); break;
+    case 0: pick<0, NT>(stage, tt, f); break;
+    case 1: pick<1, NT>(stage, tt, f); break;
+    case 2: pick<2, NT>(stage, tt, f); break;
+    case 3: pick<3, NT>(stage, tt, f); break;
+    default: pick<4, NT>(stage, tt, f); break;
   }
   return f;
+}
+
+sweep_fn sweep_kernel(int lim, int stage, int tt, int nt) {
+  return nt ? sweep_kernel_nt<1>(lim, stage, tt) : sweep_kernel_nt<0>(lim, stage, tt);
 }
 
 unsigned sign_mask(const double *s9) {
@@ -339,6 +361,7 @@ struct omhd_handle {
   bool have_state = false;
   unsigned sxl = 0, sxh = 0, syl = 0, syh = 0;
   int rows = 32;
+  int nt_store = 0;  // non-temporal stores of the stage outputs
   // optional per-stage timing with HIP events on the handle's stream
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -371,7 +394,8 @@ extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
   h->pitch = (cfg->nx + 7) & ~7;  // 64-byte aligned rows
   h->plane = (long long)h->pitch * cfg->ny;
   h->plane = (h->plane + 63) & ~63LL;
-  h->rows = cfg->rows_per_wave > 0 ? cfg->rows_per_wave : 32;
+  h->rows = cfg->rows_per_wave > 0 ? cfg->rows_per_wave : 24;
+  h->nt_store = cfg->store_hint == 1;
   h->sxl = sign_mask(cfg->bc_sign + 0);
   h->sxh = sign_mask(cfg->bc_sign + 9);
   h->syl = sign_mask(cfg->bc_sign + 18);
@@ -488,8 +512,8 @@ extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_t
   p.vmax_bits = h->red;
   p.err = (int *)(h->red + 1);
   const unsigned grid = (unsigned)((p.ntiles + 7) / 8 * 8);
-  sweep_fn s1 = sweep_kernel(c.lm_type, 1, c.time_type);
-  sweep_fn s2 = sweep_kernel(c.lm_type, 2, c.time_type);
+  sweep_fn s1 = sweep_kernel(c.lm_type, 1, c.time_type, h->nt_store);
+  sweep_fn s2 = sweep_kernel(c.lm_type, 2, c.time_type, h->nt_store);
   const double cr = 0.2;  // glm_ss2.f90:18
   for (int n = 0; n < nsteps; n++) {
     const double ch = h->vmax;                // set_dt returns vmax as ch (main.f90:80)

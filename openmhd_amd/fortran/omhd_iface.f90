@@ -25,8 +25,8 @@ module omhd_iface
      integer(c_int32_t) :: lm_type, flux_type, time_type
      integer(c_int32_t) :: bc_type(6)
      real(c_double) :: bc_sign(54)
-     integer(c_int32_t) :: rows_per_wave
-     integer(c_int32_t) :: reserved(7)
+     integer(c_int32_t) :: rows_per_wave, store_hint
+     integer(c_int32_t) :: reserved(6)
   end type omhd_config
 
   interface

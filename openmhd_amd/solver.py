@@ -21,7 +21,7 @@ class Solver2D:
     """
 
     def __init__(self, nx, ny, dx, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic", "periodic"),
-                 wall_sign=KH_WALL_SIGN, rows_per_wave=0):
+                 wall_sign=KH_WALL_SIGN, rows_per_wave=0, store_hint=0):
         self.nx, self.ny, self.dx = nx, ny, dx
         cfg = Config()
         cfg.ndim, cfg.nx, cfg.ny, cfg.nz = 2, nx, ny, 1
@@ -35,6 +35,7 @@ class Solver2D:
                 for m in range(9):
                     cfg.bc_sign[9 * f + m] = wall_sign[m] if kind == "wall" else 1.0
         cfg.rows_per_wave = rows_per_wave
+        cfg.store_hint = store_hint
         self.cfg = cfg
         h = C.c_void_p()
         check(lib.omhd_create(C.byref(cfg), C.byref(h)), "omhd_create")

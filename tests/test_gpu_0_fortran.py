@@ -1,0 +1,59 @@
+"""The Fortran side of the boundary: the project's reference-style driver
+(openmhd_amd/fortran/main2d.f90) linked against the libomhd shims, and the
+resident-step driver, both against the oracle on the same snapshot.  Bitwise."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from omhd_testlib import make_problem
+from test_oracle_golden import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+REPO = Path(__file__).resolve().parents[1]
+FDIR = REPO / "openmhd_amd" / "fortran"
+
+
+def _snapshot(tmp_path, oracle, kh, n):
+    # ghosts are filled on the CPU: this module runs the drivers as child
+    # processes before anything in this process has touched the GPU
+    from openmhd_amd import problems, snapshot
+
+    p = make_problem(n + 2, n + 2, bc=("periodic", "wall" if kh else "periodic", "periodic"))
+    if kh:
+        U, dx = problems.kelvin_helmholtz(n, n, seed=3)
+        x = snapshot.grid(n, dx, 0.0)
+        y = snapshot.grid(n, dx, -0.5 * n * dx)
+    else:
+        U, dx = problems.orszag_tang(n)
+        x = snapshot.grid(n, dx, 0.0)
+        y = x.copy()
+    oracle.bc_u(U, p)
+    f = tmp_path / "in.dat"
+    snapshot.write(f, 0.0, x, y, U)
+    return f, U, x[1] - x[0]
+
+
+@pytest.mark.parametrize("exe", ("main2d_gpu", "main2d_resident"))
+@pytest.mark.parametrize("kh", (False, True))
+def test_driver_matches_oracle(tmp_path, oracle, exe, kh):
+    from openmhd_amd import snapshot
+
+    n, steps = 40, 12
+    fin, U, dx = _snapshot(tmp_path, oracle, kh, n)
+    out, tr = tmp_path / "out.dat", tmp_path / "trace.bin"
+    cmd = [str(FDIR / exe), "--kh" if kh else "--ot", str(n), str(steps), "--input", str(fin), "--output", str(out),
+           "--trace", str(tr)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, _, Ug, _ = snapshot.read(out)
+    dts = np.fromfile(tr)[:steps]
+    p = make_problem(n + 2, n + 2, dx=dx, cfl=0.4, lm_type=1 if kh else 2,
+                     bc=("periodic", "wall" if kh else "periodic", "periodic"))
+    Uo = U.copy(order="F")
+    rc, dto, _ = oracle.run(p, Uo, steps)
+    assert rc == 0
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
