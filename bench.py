@@ -91,7 +91,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
-    ap.add_argument("--nt", type=int, default=0, help="1: non-temporal stores of the stage outputs")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -129,8 +128,7 @@ def main():
     n = a.n
     log(f"building KH {n}x{n} initial state on the host")
     U, dx = kelvin_helmholtz(n, n, seed=1)
-    s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows,
-                 store_hint=a.nt)
+    s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows)
     s.upload(U)
     del U
     log(f"warmup {a.warmup} steps")

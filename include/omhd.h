@@ -104,7 +104,7 @@ int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int
 /* ---- resident step ------------------------------------------------------ */
 
 typedef struct omhd_config {
-  int32_t ndim;         /* 2 (3D is not built yet)                          */
+  int32_t ndim;         /* 2 or 3                                           */
   int32_t nx, ny, nz;   /* interior cells (the reference's ix-2, jx-2, kx-2) */
   double dx;            /* cell size, dx == dy (rk.f90:4-5)                 */
   double cfl;           /* CFL number (main.f90:15)                         */
@@ -114,8 +114,12 @@ typedef struct omhd_config {
   int32_t bc_type[6];   /* OMHD_BC_* per face: x-, x+, y-, y+, z-, z+       */
   double bc_sign[54];   /* mirror sign per face and slot (6 x 9)            */
   int32_t rows_per_wave;/* 0: default; sweep chunk height (tuning)          */
-  int32_t store_hint;   /* 0: plain stores, 1: non-temporal stores (tuning) */
-  int32_t reserved[6];
+  int32_t resistive;    /* 3D: add the resistive flux of flux_resistive.cuf
+                           with eta = eta0 + eta1*exp(-(x^2+y^2)/eta_w^2) at
+                           the face centres, and the resistive dt limit      */
+  double x0, y0;        /* lower domain corner (cell centres x0 + (i+1/2)dx) */
+  double eta0, eta1, eta_w;
+  int32_t reserved[8];
 } omhd_config;
 
 typedef struct omhd_handle omhd_handle;
@@ -152,6 +156,29 @@ int omhd_destroy(omhd_handle *h);
  * the summed milliseconds and launch counts of stage 1 and stage 2. */
 int omhd_set_timing(omhd_handle *h, int32_t enable);
 int omhd_get_timing(omhd_handle *h, double *stage_ms, int64_t *launches);
+
+/* ---- ensemble of independent 2D grids ------------------------------------ */
+/* No counterpart routine in the reference (each reference run is one grid):
+ * n independent 2D grids, each with its own extents, boundaries and CFL dt,
+ * advanced together by one persistent launch per RK stage over a tile table.
+ * Members share lm_type and time_type.  In host dt mode (default) each
+ * member's dt and GLM factor are computed on the host exactly as omhd_step
+ * computes them, so every member is bitwise equal to its solo run; in device
+ * dt mode they are computed on the GPU (device exp) and whole steps can be
+ * captured in a hipGraph (use_graph). */
+typedef struct omhd_batch omhd_batch;
+int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch **b);
+int omhd_batch_upload(omhd_batch *b, int32_t member, const double *U);
+int omhd_batch_download(omhd_batch *b, int32_t member, double *U, double *V);
+int omhd_batch_set_mode(omhd_batch *b, int32_t device_dt, int32_t use_graph);
+/* dt_trace (optional): nsteps x n doubles, member fastest */
+int omhd_batch_step(omhd_batch *b, int32_t nsteps, double *dt_trace);
+int omhd_batch_time(omhd_batch *b, double *t);
+/* Times trial_steps steps per candidate tile height x launch shape (one
+ * workgroup per tile, or a persistent grid) and keeps the fastest; the
+ * ensemble state is restored after every trial. */
+int omhd_batch_autotune(omhd_batch *b, int32_t trial_steps, int32_t *rows, int32_t *grid);
+int omhd_batch_destroy(omhd_batch *b);
 
 /* Library version and the gfx target it was built for. */
 const char *omhd_version(void);

@@ -41,7 +41,9 @@ class Config(C.Structure):
         ("dx", C.c_double), ("cfl", C.c_double),
         ("lm_type", I32), ("flux_type", I32), ("time_type", I32),
         ("bc_type", I32 * 6), ("bc_sign", C.c_double * 54),
-        ("rows_per_wave", I32), ("store_hint", I32), ("reserved", I32 * 6),
+        ("rows_per_wave", I32), ("resistive", I32),
+        ("x0", C.c_double), ("y0", C.c_double), ("eta0", C.c_double), ("eta1", C.c_double), ("eta_w", C.c_double),
+        ("reserved", I32 * 8),
     ]
 
 
@@ -77,6 +79,14 @@ def _load() -> C.CDLL:
         "omhd_state_layout": [C.c_void_p, C.POINTER(DP), C.POINTER(C.c_int64), C.POINTER(I32)],
         "omhd_destroy": [C.c_void_p],
         "omhd_set_timing": [C.c_void_p, I32],
+        "omhd_batch_create": [C.POINTER(Config), I32, C.POINTER(C.c_void_p)],
+        "omhd_batch_upload": [C.c_void_p, I32, DP],
+        "omhd_batch_download": [C.c_void_p, I32, DP, DP],
+        "omhd_batch_set_mode": [C.c_void_p, I32, I32],
+        "omhd_batch_step": [C.c_void_p, I32, DP],
+        "omhd_batch_time": [C.c_void_p, DP],
+        "omhd_batch_autotune": [C.c_void_p, I32, C.POINTER(I32), C.POINTER(I32)],
+        "omhd_batch_destroy": [C.c_void_p],
         "omhd_get_timing": [C.c_void_p, DP, C.POINTER(C.c_int64)],
     }
     for name, args in sig.items():

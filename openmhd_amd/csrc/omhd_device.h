@@ -24,8 +24,13 @@ constexpr double HLLG_FACTOR = 1.001;       // flux_solver.f90:54
 
 // Fortran SIGN(a,b): |a| with the sign bit of b (copysign semantics, -0 -> -)
 __device__ __forceinline__ double fsign(double a, double b) { return __builtin_copysign(__builtin_fabs(a), b); }
-__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
+// MIN/MAX as the reference evaluates them.  They are only applied where the
+// operands are ordered (no NaN in a valid state) and where a zero's sign
+// cannot reach the result (absolute values in the limiters; the fan speeds
+// min/max(vL, vR) -/+ cf of flux_solver.f90:408-410), so the single-instruction
+// v_min_f64 / v_max_f64 give the reference's bits.
+__device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
 
 // ---------------------------------------------------------------------------
 // Wave-level lane shifts (64-wide wavefront, DPP wave_shr/wave_shl by 1).

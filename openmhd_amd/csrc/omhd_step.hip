@@ -1,28 +1,34 @@
-// omhd_step.hip -- the resident 2D step: fused sweep kernels + handle API.
+// omhd_step.hip -- the resident step: fused march kernels + handle API.
 //
-// One TVD-RK2 step (2D_basic/main.f90:56-188) is two launches:
+// 2D (2D_basic/main.f90:56-188): one TVD-RK2 step is two launches,
 //   stage 1: U* = U + dt/dx * (dF + dG)              (U with psi *= f on load)
 //   stage 2: U' = f_psi * 0.5 * ((U + U*) + dt/dx * (dF + dG)), plus the CFL
-//            speed of U' (the next step's set_dt), reduced to one max.
+//            speed of U' (the next step's set_dt), reduced to one max,
 // where every flux difference comes from u2v -> limiter -> HLLD on the fly.
+// 3D (3D_basic/main.f90, 3D_reconnection_gpu/main.cuf): each stage is an XY
+// pass that writes the partial difference ((F(i-1)-F(i)) + G(j-1)) - G(j) to a
+// scratch state, then a Z pass that adds H(k-1) - H(k) in the reference's
+// order and applies the RK combine (+ GLM, + CFL).  The resistive flux
+// (flux_resistive.cuf:86-135) is added at each interface when configured.
 //
-// Column march: one 64-lane wavefront owns a strip of 60 interior columns
-// (lanes 2..61; lanes 0,1,62,63 carry the two-cell x halo) and a chunk of rows.
-// It walks the chunk bottom to top, loading each row of the state exactly once
-// (coalesced 512-byte row segments per variable), and keeps a three-row window
-// of primitives in registers.  Per row it computes the y-slopes of the middle
-// row, the y-interface flux below it, the x-slopes / x-interface fluxes of the
-// oldest row with DPP lane shifts, and writes that row's update.  Boundary
-// conditions are index wraps (periodic) or mirrors with sign tables, and the
-// wall face states are mirrors of the inside face state, as the reference's
-// face patches make them (main.f90:101-102,119-120; 2D_KH/bc.f90:42-83).
+// March: one 64-lane wavefront owns a strip of 60 interior columns (lanes
+// 2..61; lanes 0,1,62,63 carry the two-cell x halo) and a chunk of rows of the
+// march axis (y, or z), at one fixed index of the remaining axis.  It walks
+// the chunk, loading each row of the state once (coalesced 512-byte segments
+// per variable), keeps a three-row ring of primitives in LDS, computes the
+// march-axis slopes and interface flux, and (XY passes) the x-slopes and
+// x-interface fluxes of the oldest row.  Boundary conditions are index wraps
+// (periodic) or mirrors with sign tables; wall face states are mirrors of the
+// inside face state, as the reference's face patches make them
+// (main.f90:101-102,119-120; 2D_KH/bc.f90:42-83).
 //
-// State layout in HBM: 9 planes (SoA), interior cells only, rows of `pitch`
-// doubles, x fastest: state[m * plane + j * pitch + i].
+// State layout in HBM: 9 planes (SoA), interior cells only, x fastest, rows of
+// `pitch` doubles, z-planes of `zstride` doubles:
+//   state[m * plane + k * zstride + j * pitch + i].
 #include <math.h>
 #include <stdlib.h>
 
-#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "omhd_device.h"
@@ -33,19 +39,32 @@ namespace {
 
 constexpr int STRIP = 60;  // interior columns per wavefront (64 lanes - 2x2 halo)
 
-struct Sweep2D {
-  const double *us;  // state whose fluxes are computed (stage 1: U^n, stage 2: U*)
-  const double *u0;  // stage 2: U^n, the RK partner
-  double *out;       // stage 1: U*, stage 2: U^{n+1} (may alias u0)
-  long long plane;   // doubles between variable planes
-  int pitch;         // doubles between rows
-  int nx, ny;
-  int nstrip, rows, ntiles;
-  int bcx, bcy;                        // OMHD_BC_* for both faces of the axis
-  unsigned sx_lo, sx_hi, sy_lo, sy_hi; // negated slots of mirror faces
-  double dtx, glm_f, ch, ch2;
-  unsigned long long *vmax_bits;       // stage 2: max speed of U^{n+1}
-  int *err;                            // [code, i, j] of the first failure
+enum Pass { PASS_2D = 0, PASS_XY = 1, PASS_Z = 2 };
+
+struct Sweep {
+  const double *us;   // state whose fluxes are computed (stage 1: U^n, stage 2: U*)
+  const double *u0;   // stage 2: U^n, the RK partner
+  double *out;        // stage 1: U*, stage 2: U^{n+1} (may alias u0)
+  double *acc;        // 3D: partial flux difference written by XY, read by Z
+  long long plane;    // doubles between variable planes
+  long long zstride;  // doubles between z-planes
+  int pitch;          // doubles between rows
+  int nx, ny, nz;
+  int nstrip, rows, nchunk, ntiles;
+  int bc[3];                  // OMHD_BC_* per axis (both faces)
+  unsigned slo[3], shi[3];    // negated slots of mirror faces, per axis
+  double dtx, glm_f, ch, ch2, dx;
+  const double *eta;          // resistive: 3 face arrays (ix*jx each) or null
+  unsigned long long *vmax_bits;  // stage 2: max speed of U^{n+1}
+  int *err;                       // [code, i, j, k] of the first failure
+  const double *dpar;  // device-side step parameters (DevPar) or null: use the fields above
+  double *U, *U1;      // ensemble members: their two state buffers
+};
+
+// Step parameters computed on the device from the reduced CFL speed (device
+// dt mode: no host round trip, so whole steps can be captured in a graph).
+struct DevPar {
+  double ch, ch2, dtx1, dtx2, glm_f, dt;
 };
 
 // consecutive workgroups go round-robin to the 8 XCDs: hand each XCD a
@@ -66,42 +85,80 @@ __device__ __forceinline__ void map_axis(int c, int n, int bc, unsigned slo, uns
   cm = cm < 0 ? 0 : (cm > n - 1 ? n - 1 : cm);  // lanes past the grid: any valid cell
 }
 
-__device__ __forceinline__ double flip(double v, unsigned mask, int m) { return ((mask >> m) & 1u) ? -v : v; }
+// negate slot m when bit m of mask is set: a sign-bit XOR (exact, like the
+// reference's U(:,1,my) = -U(:,2,my))
+__device__ __forceinline__ double flip(double v, unsigned mask, int m) {
+  const long long sbit = (long long)((mask >> m) & 1u) << 63;
+  return __longlong_as_double(__double_as_longlong(v) ^ sbit);
+}
 
-__device__ __forceinline__ void record_error(int *err, int code, int i, int j) {
+__device__ __forceinline__ void record_error(int *err, int code, int i, int j, int k) {
   if (atomicCAS(err, 0, code) == 0) {
     err[1] = i;
     err[2] = j;
+    err[3] = k;
   }
 }
 
 // Per wavefront LDS: a ring of three primitive rows (rn-2, rn-1, rn) and the
-// carried left face state of the y-interface above row rn-2.  Keeping these
-// out of VGPRs leaves ~18 live doubles across each inlined HLLD call, so two
-// wavefronts fit per SIMD.
+// carried left face state of the march-axis interface above row rn-2; for the
+// resistive flux also the transverse B samples of each ring row and Bx of the
+// row below the ring.  Keeping these out of VGPRs leaves ~18 live doubles
+// across each inlined HLLD call, so two wavefronts fit per SIMD.
 struct SweepLds {
   double ring[3][NV][66];  // [slot][var][lane + 1]; entries 0 and 65 repeat lanes 0 and 63
   double wly[NV][64];
 };
+struct SweepLdsRes {
+  double ring[3][NV][66];
+  double wly[NV][64];
+  double aux[3][4][66];  // XY: Bx(k-1), Bx(k+1), By(k-1), By(k+1); Z: Bz(j-1), Bz(j+1)
+  double bxm[66];        // XY: Bx of row rn-3
+};
 
-template <int LIM, int STAGE, int TT, int NT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_sweep2d(const Sweep2D p) {
-  __shared__ SweepLds lds;
+__device__ __forceinline__ void put66(double *row, int lane, double v) {
+  row[lane + 1] = v;
+  if (lane == 0) row[0] = v;
+  if (lane == 63) row[65] = v;
+}
+
+template <int RES>
+using LdsOf = typename std::conditional<RES != 0, SweepLdsRes, SweepLds>::type;
+
+// One tile: strip x chunk (x slice) of one grid.
+template <int LIM, int STAGE, int TT, int PASS, int RES>
+__device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RES> &lds) {
+  constexpr bool XS = PASS != PASS_Z;         // x-sweep of the oldest row
+  constexpr bool FINAL = PASS != PASS_XY;     // RK update (else write acc)
+  constexpr int MD = PASS == PASS_Z ? 2 : 1;  // march axis = direction of its HLLD
+  constexpr int ND = PASS == PASS_2D ? 2 : 3;
   const int lane = threadIdx.x;
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  if (t >= p.ntiles) return;
-  const int strip = t % p.nstrip, chunk = t / p.nstrip;
+  const int strip = t % p.nstrip;
+  const int r2 = t / p.nstrip;
+  const int chunk = r2 % p.nchunk, slice = r2 / p.nchunk;  // slice: k (XY) or j (Z)
   const int c = strip * STRIP - 2 + lane;
+  const int nm = MD == 1 ? p.ny : p.nz;  // cells along the march axis
   const int j0 = chunk * p.rows;
-  const int hc = min(p.rows, p.ny - j0);
+  const int hc = min(p.rows, nm - j0);
+  const long long rstride = MD == 1 ? (long long)p.pitch : p.zstride;
+  const long long fixoff = PASS == PASS_Z ? (long long)slice * p.pitch : (long long)slice * p.zstride;
   int cm;
   unsigned xmask;
-  map_axis(c, p.nx, p.bcx, p.sx_lo, p.sx_hi, cm, xmask);
+  map_axis(c, p.nx, p.bc[0], p.slo[0], p.shi[0], cm, xmask);
   const bool writer = lane >= 2 && lane < 2 + STRIP && c < p.nx;
-  const bool xlo_face = p.bcx == OMHD_BC_MIRROR && c == -1;
-  const bool xhi_face = p.bcx == OMHD_BC_MIRROR && c == p.nx - 1;
-  const double glm_us = (STAGE == 1) ? p.glm_f : 1.0;
+  const bool xlo_face = p.bc[0] == OMHD_BC_MIRROR && c == -1;
+  const bool xhi_face = p.bc[0] == OMHD_BC_MIRROR && c == p.nx - 1;
+  double ch = p.ch, ch2 = p.ch2, dtx = p.dtx, glm_f = p.glm_f;
+  if (p.dpar) {
+    const DevPar *dp = (const DevPar *)p.dpar;
+    ch = dp->ch;
+    ch2 = dp->ch2;
+    dtx = STAGE == 1 ? dp->dtx1 : dp->dtx2;
+    glm_f = dp->glm_f;
+  }
+  const double glm_us = (STAGE == 1) ? glm_f : 1.0;
   const long long plane = p.plane;
+  const int ix = p.nx + 2, jx = p.ny + 2;
 
   double Gl[NV], nxt[NV];
   double vloc = 0.0;
@@ -109,9 +166,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   auto row_addr = [&](int rn, unsigned &mask) -> long long {
     int rm;
     unsigned ymask;
-    map_axis(rn, p.ny, p.bcy, p.sy_lo, p.sy_hi, rm, ymask);
+    map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ymask);
     mask = xmask ^ ymask;
-    return (long long)rm * p.pitch + cm;
+    return fixoff + (long long)rm * rstride + cm;
   };
 
   unsigned mask_nxt;
@@ -130,51 +187,95 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       for (int m = 0; m < NV; m++) u[m] = flip(nxt[m], mask_nxt, m);
       u[8] = glm_us * u[8];  // glm_ss2 before the sweep (main.f90:87)
       if (k + 1 < hc + 4) {  // prefetch the next row
-        const long long o = row_addr(rn + 1, mask_nxt);
+        unsigned mnext;
+        const long long o = row_addr(rn + 1, mnext);
 #pragma unroll
         for (int m = 0; m < NV; m++) nxt[m] = p.us[m * plane + o];
+        mask_nxt = mnext;
       }
       cons_to_prim(u, W);  // u2v
       if (!(W[3] > 0.0) || !(u[4] > 0.0)) {
         int rm;
         unsigned dummy;
-        map_axis(rn, p.ny, p.bcy, p.sy_lo, p.sy_hi, rm, dummy);
-        record_error(p.err, (W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, cm + 2, rm + 2);
+        map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, dummy);
+        const int jj = MD == 1 ? rm : slice;
+        const int kk = MD == 1 ? (ND == 3 ? slice + 2 : 0) : rm + 2;
+        record_error(p.err, (W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, cm + 2, jj + 2, kk);
+      }
+      if constexpr (RES != 0) {
+        // transverse B samples of this row for the resistive face currents
+        int rm;
+        unsigned ymask;
+        map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ymask);
+        if constexpr (PASS == PASS_XY) {
+          put66(lds.bxm, lane, lds.ring[sc][5][lane + 1]);  // Bx of row rn-3, before its slot is reused
+#pragma unroll
+          for (int s2 = 0; s2 < 2; s2++) {
+            int km;
+            unsigned zmask;
+            map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zmask);
+            const unsigned mk = xmask ^ ymask ^ zmask;
+            const long long o = (long long)km * p.zstride + (long long)rm * p.pitch + cm;
+            put66(lds.aux[sc][s2], lane, flip(p.us[5 * plane + o], mk, 5));
+            put66(lds.aux[sc][2 + s2], lane, flip(p.us[6 * plane + o], mk, 6));
+          }
+        } else {
+#pragma unroll
+          for (int s2 = 0; s2 < 2; s2++) {
+            int jm;
+            unsigned jmask;
+            map_axis(slice + (s2 ? 1 : -1), p.ny, p.bc[1], p.slo[1], p.shi[1], jm, jmask);
+            const unsigned mk = xmask ^ ymask ^ jmask;
+            const long long o = (long long)rm * p.zstride + (long long)jm * p.pitch + cm;
+            put66(lds.aux[sc][s2], lane, flip(p.us[7 * plane + o], mk, 7));
+          }
+        }
       }
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        lds.ring[sc][m][lane + 1] = W[m];
-        if (lane == 0) lds.ring[sc][m][0] = W[m];
-        if (lane == 63) lds.ring[sc][m][65] = W[m];
-      }
+      for (int m = 0; m < NV; m++) put66(lds.ring[sc][m], lane, W[m]);
     }
     __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
 
     // x-sweep of row r = rn-2: tq = (F(i-1) - F(i)) + G(j-1)   (rk.f90:30-31 order)
     double tq[NV];
-    if (k >= 4) {
-      double Lx[NV], Rx[NV], F[NV];
+    if constexpr (XS) {
+      if (k >= 4) {
+        double Lx[NV], Rx[NV], F[NV];
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        double wRx;
-        limit<LIM>(lds.ring[sa][m][lane], lds.ring[sa][m][lane + 1], lds.ring[sa][m][lane + 2], wRx, Lx[m]);
-        Rx[m] = from_next(wRx);
+        for (int m = 0; m < NV; m++) {
+          double wRx;
+          limit<LIM>(lds.ring[sa][m][lane], lds.ring[sa][m][lane + 1], lds.ring[sa][m][lane + 2], wRx, Lx[m]);
+          Rx[m] = from_next(wRx);
+        }
+        if (xlo_face) {
+#pragma unroll
+          for (int m = 0; m < NV; m++) Lx[m] = flip(Rx[m], p.slo[0], m);
+        }
+        if (xhi_face) {
+#pragma unroll
+          for (int m = 0; m < NV; m++) Rx[m] = flip(Lx[m], p.shi[0], m);
+        }
+        hlld<0>(Lx, Rx, ch, ch2, F);
+        if constexpr (RES != 0) {  // flux_resistive.cuf:89-101 at x-face (c+1/2, row rn-2, plane slice)
+          const int l = lane + 1;
+          const double *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
+          const double *B5 = lds.ring[sb][5];
+          const double *kp = lds.aux[sa][1], *km = lds.aux[sa][0];
+          const double JyS = -(A7[l + 1] - A7[l]) + 0.25 * (kp[l] + kp[l + 1] - km[l] - km[l + 1]);
+          const double JzS = (A6[l + 1] - A6[l]) - 0.25 * (B5[l] + B5[l + 1] - lds.bxm[l] - lds.bxm[l + 1]);
+          const int cf = min(max(c + 1, 0), ix - 1);  // lanes outside the face range read a valid entry
+          const double eta_dx = p.eta[(size_t)cf + (size_t)ix * (rn - 1)] / p.dx;
+          F[3] = F[3] + 0.5 * eta_dx * (JyS * (A7[l] + A7[l + 1]) - JzS * (A6[l] + A6[l + 1]));
+          F[6] = F[6] - eta_dx * JzS;
+          F[7] = F[7] + eta_dx * JyS;
+        }
+#pragma unroll
+        for (int m = 0; m < NV; m++) tq[m] = from_prev(F[m]) - F[m] + Gl[m];
       }
-      if (xlo_face) {
-#pragma unroll
-        for (int m = 0; m < NV; m++) Lx[m] = flip(Rx[m], p.sx_lo, m);
-      }
-      if (xhi_face) {
-#pragma unroll
-        for (int m = 0; m < NV; m++) Rx[m] = flip(Lx[m], p.sx_hi, m);
-      }
-      hlld<0>(Lx, Rx, p.ch, p.ch2, F);
-#pragma unroll
-      for (int m = 0; m < NV; m++) tq[m] = from_prev(F[m]) - F[m] + Gl[m];
     }
 
     if (k >= 2) {
-      // y-slopes of row rn-1 from rows rn-2, rn-1, rn
+      // march-axis slopes of row rn-1 from rows rn-2, rn-1, rn
       double Lf[NV], Rf[NV];
 #pragma unroll
       for (int m = 0; m < NV; m++) {
@@ -184,53 +285,83 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         lds.wly[m][lane] = wLyB;
       }
       if (k >= 3) {
-        // y-interface between rows rn-2 and rn-1
-        if (p.bcy == OMHD_BC_MIRROR) {
-          if (rn - 1 == 0) {  // south wall: bc_for_G, 2D_KH/bc.f90:62-70
+        // march-axis interface between rows rn-2 and rn-1
+        if (p.bc[MD] == OMHD_BC_MIRROR) {
+          if (rn - 1 == 0) {  // low wall: bc_for_G, 2D_KH/bc.f90:62-70
 #pragma unroll
-            for (int m = 0; m < NV; m++) Lf[m] = flip(Rf[m], p.sy_lo, m);
+            for (int m = 0; m < NV; m++) Lf[m] = flip(Rf[m], p.slo[MD], m);
           }
-          if (rn - 1 == p.ny) {  // north wall: 2D_KH/bc.f90:73-81
+          if (rn - 1 == nm) {  // high wall: 2D_KH/bc.f90:73-81
 #pragma unroll
-            for (int m = 0; m < NV; m++) Rf[m] = flip(Lf[m], p.sy_hi, m);
+            for (int m = 0; m < NV; m++) Rf[m] = flip(Lf[m], p.shi[MD], m);
           }
         }
         double G[NV];
-        hlld<1>(Lf, Rf, p.ch, p.ch2, G);
+        hlld<MD>(Lf, Rf, ch, ch2, G);
+        if constexpr (RES != 0) {
+          const int l = lane + 1;
+          const double *A5 = lds.ring[sa][5], *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
+          const double *B5 = lds.ring[sb][5], *B6 = lds.ring[sb][6], *B7 = lds.ring[sb][7];
+          if constexpr (PASS == PASS_XY) {  // flux_resistive.cuf:106-118, y-face (c, rn-2 + 1/2, slice)
+            const double *akp = lds.aux[sa][3], *akm = lds.aux[sa][2], *bkp = lds.aux[sb][3], *bkm = lds.aux[sb][2];
+            const double JxS = (B7[l] - A7[l]) - 0.25 * (akp[l] + bkp[l] - akm[l] - bkm[l]);
+            const double JzS = -(B5[l] - A5[l]) + 0.25 * (A6[l + 1] + B6[l + 1] - A6[l - 1] - B6[l - 1]);
+            const double eta_dx = p.eta[(size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * (rn - 1)] / p.dx;
+            G[3] = G[3] + 0.5 * eta_dx * (JzS * (A5[l] + B5[l]) - JxS * (A7[l] + B7[l]));
+            G[5] = G[5] + eta_dx * JzS;
+            G[7] = G[7] - eta_dx * JxS;
+          } else {  // flux_resistive.cuf:123-135, z-face (c, slice, rn-2 + 1/2)
+            const double *ajp = lds.aux[sa][1], *ajm = lds.aux[sa][0], *bjp = lds.aux[sb][1], *bjm = lds.aux[sb][0];
+            const double JyS = (B5[l] - A5[l]) - 0.25 * (A7[l + 1] + B7[l + 1] - A7[l - 1] - B7[l - 1]);
+            const double JxS = -(B6[l] - A6[l]) + 0.25 * (ajp[l] + bjp[l] - ajm[l] - bjm[l]);
+            const double eta_dx = p.eta[2 * (size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * (slice + 1)] / p.dx;
+            G[3] = G[3] + 0.5 * eta_dx * (JxS * (A6[l] + B6[l]) - JyS * (A5[l] + B5[l]));
+            G[5] = G[5] - eta_dx * JyS;
+            G[6] = G[6] + eta_dx * JxS;
+          }
+        }
 
         if (k >= 4 && writer) {
           const int r = rn - 2;
-          const long long o = (long long)r * p.pitch + c;
-          double o9[NV];
-          if constexpr (STAGE == 1) {
+          const long long o = fixoff + (long long)r * rstride + c;
+          double d[NV];
+          if constexpr (PASS == PASS_2D) {
 #pragma unroll
-            for (int m = 0; m < NV; m++) {
-              double uu = p.us[m * plane + o];
-              if (m == 8) uu = p.glm_f * uu;
-              o9[m] = uu + p.dtx * (tq[m] - G[m]);  // rk_tvd21 / rk_std21
-            }
+            for (int m = 0; m < NV; m++) d[m] = tq[m] - G[m];
+          } else if constexpr (PASS == PASS_XY) {
+#pragma unroll
+            for (int m = 0; m < NV; m++) p.acc[m * plane + o] = tq[m] - G[m];  // ((F-F)+G)-G
           } else {
 #pragma unroll
-            for (int m = 0; m < NV; m++) {
-              double a0 = p.u0[m * plane + o];
-              if (m == 8) a0 = p.glm_f * a0;
-              if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + p.dtx * (tq[m] - G[m]));  // rk_tvd22
-              else o9[m] = a0 + p.dtx * (tq[m] - G[m]);                                                   // rk_std22
-            }
-            o9[8] = p.glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
+            for (int m = 0; m < NV; m++) d[m] = p.acc[m * plane + o] + Gl[m] - G[m];  // (.. + H(k-1)) - H(k)
           }
-          if constexpr (NT) {  // streaming store: the next launch reads it, this one does not
+          if constexpr (FINAL) {
+            double o9[NV];
+            if constexpr (STAGE == 1) {
 #pragma unroll
-            for (int m = 0; m < NV; m++) __builtin_nontemporal_store(o9[m], &p.out[m * plane + o]);
-          } else {
+              for (int m = 0; m < NV; m++) {
+                double uu = p.us[m * plane + o];
+                if (m == 8) uu = glm_f * uu;
+                o9[m] = uu + dtx * d[m];  // rk_tvd21 / rk_std21
+              }
+            } else {
+#pragma unroll
+              for (int m = 0; m < NV; m++) {
+                double a0 = p.u0[m * plane + o];
+                if (m == 8) a0 = glm_f * a0;
+                if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + dtx * d[m]);  // rk_tvd22
+                else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
+              }
+              o9[8] = glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
+            }
 #pragma unroll
             for (int m = 0; m < NV; m++) p.out[m * plane + o] = o9[m];
-          }
-          if constexpr (STAGE == 2) {  // next step's set_dt on U^{n+1}
-            double w2[NV];
-            cons_to_prim(o9, w2);
-            const double vt = fastest_speed<2>(o9, w2);
-            vloc = (vt > vloc || !(vt == vt)) ? vt : vloc;
+            if constexpr (STAGE == 2) {  // next step's set_dt on U^{n+1}
+              double w2[NV];
+              cons_to_prim(o9, w2);
+              const double vt = fastest_speed<ND>(o9, w2);
+              vloc = (vt > vloc || !(vt == vt)) ? vt : vloc;
+            }
           }
         }
 #pragma unroll
@@ -238,29 +369,88 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       }
     }
   }
-  if constexpr (STAGE == 2) {
-    if (!(vloc >= 0.0)) record_error(p.err, OMHD_ERR_DT, 0, 0);
+  if constexpr (FINAL && STAGE == 2) {
+    if (!(vloc >= 0.0)) record_error(p.err, OMHD_ERR_DT, 0, 0, 0);
     const double vm = wave_max(vloc >= 0.0 ? vloc : 0.0);
     if (lane == 0) atomic_max_nonneg(p.vmax_bits, vm);
   }
 }
 
+template <int LIM, int STAGE, int TT, int PASS, int RES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march(const Sweep p) {
+  __shared__ LdsOf<RES> lds;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  if (t >= p.ntiles) return;
+  march_tile<LIM, STAGE, TT, PASS, RES>(p, t, lds);
+}
+
+// Ensemble: one persistent launch per RK stage for every member.  The tile
+// table lists (member, tile) pairs; each workgroup takes every gridDim-th entry.
+template <int LIM, int STAGE, int TT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march_ens(
+    const Sweep *__restrict__ members, const int2 *__restrict__ tiles, int ntiles) {
+  __shared__ LdsOf<0> lds;
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const int2 tl = tiles[w];
+    Sweep p = members[tl.x];
+    p.us = STAGE == 1 ? p.U : p.U1;
+    p.u0 = p.U;
+    p.out = STAGE == 1 ? p.U1 : p.U;
+    march_tile<LIM, STAGE, TT, PASS_2D, 0>(p, tl.y, lds);
+    __syncthreads();
+  }
+}
+
+// Device dt: for each of n members, dt and the GLM factor from its reduced
+// CFL speed (set_dt.f90:52, glm_ss2.f90:27, flux_resistive.cuf:8), then the
+// speed slot is cleared for the next step's reduction.  exp is the device
+// libm's, so a run in this mode is compared with runs in the same mode.
+__global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const double *dx, const double *cfl,
+                            const int *tt, const double *eta_max, int *err, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double vmax = __longlong_as_double((long long)vmax_bits[i]);
+  double dt = cfl[i] * dx[i] / vmax;
+  if (!(dt >= 1.0e-7)) {
+    record_error(err, OMHD_ERR_DT, i, 0, 0);
+    dt = 0.0;
+  }
+  if (eta_max && eta_max[i] > 0.0) {
+    const double dr = 0.5 * cfl[i] * (dx[i] * dx[i]) / eta_max[i];
+    dt = dt < dr ? dt : dr;
+  }
+  DevPar q;
+  q.ch = vmax;
+  q.ch2 = vmax * vmax;
+  q.dtx1 = tt[i] == 0 ? dt / dx[i] : 0.5 * dt / dx[i];
+  q.dtx2 = dt / dx[i];
+  q.glm_f = exp(-0.5 * dt * vmax / 0.2);
+  q.dt = dt;
+  par[i] = q;
+  vmax_bits[i] = 0ull;
+}
+
 // CFL speed of the resident state (set_dt.f90:21-45), used once after upload
-__global__ void k_cfl2d(const double *__restrict__ s, long long plane, int pitch, int nx, int ny,
-                        unsigned long long *vmax_bits, int *err) {
+template <int ND>
+__global__ void k_cfl(const double *__restrict__ s, long long plane, long long zstride, int pitch, int nx, int ny,
+                      int nz, unsigned long long *vmax_bits, int *err) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double vt = 0.0;
-  if (gid < (long long)nx * ny) {
-    const int i = (int)(gid % nx), j = (int)(gid / nx);
-    const long long o = (long long)j * pitch + i;
+  if (gid < (long long)nx * ny * nz) {
+    const int i = (int)(gid % nx);
+    const long long r = gid / nx;
+    const int j = (int)(r % ny), k = (int)(r / ny);
+    const long long o = (long long)k * zstride + (long long)j * pitch + i;
     double u[NV], w[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) u[m] = s[m * plane + o];
     cons_to_prim(u, w);
-    if (!(w[3] > 0.0) || !(u[4] > 0.0)) record_error(err, (w[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, i + 2, j + 2);
-    vt = fastest_speed<2>(u, w);
+    const int kk = ND == 3 ? k + 2 : 0;
+    if (!(w[3] > 0.0) || !(u[4] > 0.0))
+      record_error(err, (w[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, i + 2, j + 2, kk);
+    vt = fastest_speed<ND>(u, w);
     if (!(vt >= 0.0)) {
-      record_error(err, OMHD_ERR_DT, i + 2, j + 2);
+      record_error(err, OMHD_ERR_DT, i + 2, j + 2, kk);
       vt = 0.0;
     }
   }
@@ -268,21 +458,28 @@ __global__ void k_cfl2d(const double *__restrict__ s, long long plane, int pitch
   if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax_bits, vt);
 }
 
-// Fortran array U(ix,jx,9) (+ V(ix,jx,4)) with ghosts <-> SoA interior
-__global__ void k_to_fortran(const double *__restrict__ s, long long plane, int pitch, int nx, int ny, int bcx, int bcy,
-                             unsigned sxl, unsigned sxh, unsigned syl, unsigned syh, double *__restrict__ U,
-                             double *__restrict__ V) {
-  const int ix = nx + 2, jx = ny + 2;
+// Fortran array U(ix,jx[,kx],9) (+ V(..,4)) with ghosts <- SoA interior
+struct Faces {
+  int bc[3];
+  unsigned slo[3], shi[3];
+};
+
+__global__ void k_to_fortran(const double *__restrict__ s, long long plane, long long zstride, int pitch, int nx,
+                             int ny, int nz, int nd, Faces f, double *__restrict__ U, double *__restrict__ V) {
+  const int ix = nx + 2, jx = ny + 2, kx = nd == 3 ? nz + 2 : 1;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)ix * jx) return;
-  const int i = (int)(gid % ix), j = (int)(gid / ix);
-  int im, jm;
-  unsigned mx_, my_;
-  map_axis(i - 1, nx, bcx, sxl, sxh, im, mx_);
-  map_axis(j - 1, ny, bcy, syl, syh, jm, my_);
-  const unsigned mask = mx_ ^ my_;
-  const long long o = (long long)jm * pitch + im;
-  const long long n = (long long)ix * jx;
+  const long long n = (long long)ix * jx * kx;
+  if (gid >= n) return;
+  const int i = (int)(gid % ix);
+  const long long r = gid / ix;
+  const int j = (int)(r % jx), k = (int)(r / jx);
+  int im, jm, km = 0;
+  unsigned mx_, my_, mz_ = 0;
+  map_axis(i - 1, nx, f.bc[0], f.slo[0], f.shi[0], im, mx_);
+  map_axis(j - 1, ny, f.bc[1], f.slo[1], f.shi[1], jm, my_);
+  if (nd == 3) map_axis(k - 1, nz, f.bc[2], f.slo[2], f.shi[2], km, mz_);
+  const unsigned mask = mx_ ^ my_ ^ mz_;
+  const long long o = (long long)km * zstride + (long long)jm * pitch + im;
   double u[NV], w[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) u[m] = flip(s[m * plane + o], mask, m);
@@ -295,45 +492,48 @@ __global__ void k_to_fortran(const double *__restrict__ s, long long plane, int 
   }
 }
 
-__global__ void k_from_fortran(const double *__restrict__ U, double *__restrict__ s, long long plane, int pitch, int nx,
-                               int ny) {
-  const int ix = nx + 2, jx = ny + 2;
+__global__ void k_from_fortran(const double *__restrict__ U, double *__restrict__ s, long long plane, long long zstride,
+                               int pitch, int nx, int ny, int nz, int nd) {
+  const int ix = nx + 2, jx = ny + 2, kx = nd == 3 ? nz + 2 : 1;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)nx * ny) return;
-  const int i = (int)(gid % nx), j = (int)(gid / nx);
-  const long long n = (long long)ix * jx;
-  const long long f = (long long)(j + 1) * ix + (i + 1);
+  if (gid >= (long long)nx * ny * nz) return;
+  const int i = (int)(gid % nx);
+  const long long r = gid / nx;
+  const int j = (int)(r % ny), k = (int)(r / ny);
+  const long long n = (long long)ix * jx * kx;
+  const long long fk = nd == 3 ? k + 1 : 0;
+  const long long f = (fk * jx + (j + 1)) * ix + (i + 1);
 #pragma unroll
-  for (int m = 0; m < NV; m++) s[m * plane + (long long)j * pitch + i] = U[m * n + f];
+  for (int m = 0; m < NV; m++) s[m * plane + (long long)k * zstride + (long long)j * pitch + i] = U[m * n + f];
 }
 
 __global__ void k_zero_red(unsigned long long *red) {
-  if (threadIdx.x < 3) red[threadIdx.x] = 0ull;
+  if (threadIdx.x < 4) red[threadIdx.x] = 0ull;
 }
 
-typedef void (*sweep_fn)(const Sweep2D);
+typedef void (*march_fn)(const Sweep);
 
-template <int LIM, int NT>
-void pick(int stage, int tt, sweep_fn &f) {
-  if (stage == 1) f = tt == 0 ? k_sweep2d<LIM, 1, 0, NT> : k_sweep2d<LIM, 1, 1, NT>;
-  else f = tt == 0 ? k_sweep2d<LIM, 2, 0, NT> : k_sweep2d<LIM, 2, 1, NT>;
+template <int LIM, int PASS, int RES>
+march_fn pick3(int stage, int tt) {
+  if (stage == 1) return tt == 0 ? k_march<LIM, 1, 0, PASS, RES> : k_march<LIM, 1, 1, PASS, RES>;
+  return tt == 0 ? k_march<LIM, 2, 0, PASS, RES> : k_march<LIM, 2, 1, PASS, RES>;
 }
 
-template <int NT>
-sweep_fn sweep_kernel_nt(int lim, int stage, int tt) {
-  sweep_fn f = nullptr;
+template <int PASS, int RES>
+march_fn pick2(int lim, int stage, int tt) {
   switch (lim) {
-    case 0: pick<0, NT>(stage, tt, f); break;
-    case 1: pick<1, NT>(stage, tt, f); break;
-    case 2: pick<2, NT>(stage, tt, f); break;
-    case 3: pick<3, NT>(stage, tt, f); break;
-    default: pick<4, NT>(stage, tt, f); break;
+    case 0: return pick3<0, PASS, RES>(stage, tt);
+    case 1: return pick3<1, PASS, RES>(stage, tt);
+    case 2: return pick3<2, PASS, RES>(stage, tt);
+    case 3: return pick3<3, PASS, RES>(stage, tt);
+    default: return pick3<4, PASS, RES>(stage, tt);
   }
-  return f;
 }
 
-sweep_fn sweep_kernel(int lim, int stage, int tt, int nt) {
-  return nt ? sweep_kernel_nt<1>(lim, stage, tt) : sweep_kernel_nt<0>(lim, stage, tt);
+march_fn march_kernel(int pass, int res, int lim, int stage, int tt) {
+  if (pass == PASS_2D) return pick2<PASS_2D, 0>(lim, stage, tt);
+  if (pass == PASS_XY) return res ? pick2<PASS_XY, 1>(lim, stage, tt) : pick2<PASS_XY, 0>(lim, stage, tt);
+  return res ? pick2<PASS_Z, 1>(lim, stage, tt) : pick2<PASS_Z, 0>(lim, stage, tt);
 }
 
 unsigned sign_mask(const double *s9) {
@@ -350,18 +550,21 @@ using namespace omhd;
 
 struct omhd_handle {
   omhd_config cfg;
+  int nd = 2;
   int pitch = 0;
-  long long plane = 0;
-  double *U = nullptr;   // U^n / U^{n+1}
-  double *U1 = nullptr;  // U*
-  unsigned long long *red = nullptr;  // [0] vmax bits, [1..] error (3 ints)
+  long long zstride = 0, plane = 0;
+  double *U = nullptr;    // U^n / U^{n+1}
+  double *U1 = nullptr;   // U*
+  double *acc = nullptr;  // 3D: partial flux difference
+  double *eta = nullptr;  // resistive face arrays
+  double eta_max = 0.0;
+  unsigned long long *red = nullptr;       // [0] vmax bits, [1..2] error (4 ints)
   unsigned long long *red_host = nullptr;  // pinned mirror
   hipStream_t stream = nullptr;
   double vmax = -1.0;  // CFL speed of the current state
   bool have_state = false;
-  unsigned sxl = 0, sxh = 0, syl = 0, syh = 0;
-  int rows = 32;
-  int nt_store = 0;  // non-temporal stores of the stage outputs
+  Faces faces{};
+  int rows = 24;
   // optional per-stage timing with HIP events on the handle's stream
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -369,43 +572,82 @@ struct omhd_handle {
   long long stage_launches[2] = {0, 0};
 };
 
-static int read_red(omhd_handle *h, double &vmax, int err[3]) {
+static int read_red(omhd_handle *h, double &vmax, int err[4]) {
   OMHD_HIP(hipMemcpyAsync(h->red_host, h->red, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
   OMHD_HIP(hipStreamSynchronize(h->stream));
   memcpy(&vmax, &h->red_host[0], sizeof(double));
-  memcpy(err, &h->red_host[1], 3 * sizeof(int));
+  memcpy(err, &h->red_host[1], 4 * sizeof(int));
   return OMHD_OK;
+}
+
+// the face arrays of the configured resistivity eta(x,y) = eta0 + eta1 *
+// exp(-(x^2+y^2)/w^2), at x-, y- and z-face centres, computed on the host with
+// libm (transcendentals stay off the device so the arithmetic matches a CPU
+// build); cell centres are x0 - dx/2 + i*dx accumulated as model.f90:19-28 does
+static double eta_faces(const omhd_config &c, std::vector<double> &out) {
+  const int ix = c.nx + 2, jx = c.ny + 2;
+  std::vector<double> x(ix), y(jx);
+  x[0] = c.x0 - c.dx / 2;
+  for (int i = 1; i < ix; i++) x[i] = x[i - 1] + c.dx;
+  y[0] = c.y0 - c.dx / 2;
+  for (int j = 1; j < jx; j++) y[j] = y[j - 1] + c.dx;
+  out.assign((size_t)3 * ix * jx, 0.0);
+  double emax = 0.0;
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < jx; j++)
+      for (int i = 0; i < ix; i++) {
+        const double xx = x[i] + (d == 0 ? 0.5 * c.dx : 0.0);
+        const double yy = y[j] + (d == 1 ? 0.5 * c.dx : 0.0);
+        const double r2 = xx * xx + yy * yy;
+        const double e = c.eta0 + c.eta1 * exp(-r2 / (c.eta_w * c.eta_w));
+        out[(size_t)d * ix * jx + (size_t)i + (size_t)ix * j] = e;
+        if (e > emax) emax = e;
+      }
+  return emax;
 }
 
 extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
   if (!cfg || !out) return set_error(OMHD_ERR_ARG, "null argument");
-  if (cfg->ndim != 2) return set_error(OMHD_ERR_ARG, "the resident step is built for ndim == 2", cfg->ndim);
-  if (cfg->nx < 2 || cfg->ny < 2) return set_error(OMHD_ERR_EXTENT, "need nx, ny >= 2", cfg->nx, cfg->ny);
+  if (cfg->ndim != 2 && cfg->ndim != 3) return set_error(OMHD_ERR_ARG, "ndim must be 2 or 3", cfg->ndim);
+  const int nz = cfg->ndim == 3 ? cfg->nz : 1;
+  if (cfg->nx < 2 || cfg->ny < 2 || (cfg->ndim == 3 && nz < 2))
+    return set_error(OMHD_ERR_EXTENT, "need nx, ny (, nz) >= 2", cfg->nx, cfg->ny, nz);
   if (cfg->lm_type < 0 || cfg->lm_type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", cfg->lm_type);
   if (cfg->flux_type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", cfg->flux_type);
   if (cfg->time_type != 0 && cfg->time_type != 1) return set_error(OMHD_ERR_TYPE, "unknown time_type", cfg->time_type);
   if (!(cfg->dx > 0) || !(cfg->cfl > 0)) return set_error(OMHD_ERR_ARG, "dx and cfl must be positive");
-  for (int a = 0; a < 2; a++)
-    if (cfg->bc_type[2 * a] != cfg->bc_type[2 * a + 1] &&
-        (cfg->bc_type[2 * a] == OMHD_BC_PERIODIC || cfg->bc_type[2 * a + 1] == OMHD_BC_PERIODIC))
-      return set_error(OMHD_ERR_ARG, "periodic faces come in pairs", a);
+  if (cfg->resistive && cfg->ndim != 3)
+    return set_error(OMHD_ERR_ARG, "the resistive flux is built for the 3D step (flux_resistive.cuf)");
+  if (cfg->resistive && !(cfg->eta_w > 0)) return set_error(OMHD_ERR_ARG, "eta_w must be positive");
+  for (int a = 0; a < 3; a++)
+    if (cfg->bc_type[2 * a] != cfg->bc_type[2 * a + 1])
+      return set_error(OMHD_ERR_ARG, "both faces of an axis share one boundary type", a);
   omhd_handle *h = new omhd_handle();
   h->cfg = *cfg;
+  h->cfg.nz = nz;
+  h->nd = cfg->ndim;
   h->pitch = (cfg->nx + 7) & ~7;  // 64-byte aligned rows
-  h->plane = (long long)h->pitch * cfg->ny;
-  h->plane = (h->plane + 63) & ~63LL;
+  h->zstride = (long long)h->pitch * cfg->ny;
+  h->plane = ((h->zstride * nz) + 63) & ~63LL;
   h->rows = cfg->rows_per_wave > 0 ? cfg->rows_per_wave : 24;
-  h->nt_store = cfg->store_hint == 1;
-  h->sxl = sign_mask(cfg->bc_sign + 0);
-  h->sxh = sign_mask(cfg->bc_sign + 9);
-  h->syl = sign_mask(cfg->bc_sign + 18);
-  h->syh = sign_mask(cfg->bc_sign + 27);
+  for (int a = 0; a < 3; a++) {
+    h->faces.bc[a] = cfg->bc_type[2 * a];
+    h->faces.slo[a] = sign_mask(cfg->bc_sign + 18 * a);
+    h->faces.shi[a] = sign_mask(cfg->bc_sign + 18 * a + 9);
+  }
   const size_t bytes = (size_t)h->plane * NV * sizeof(double);
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void **)&h->U, bytes);
   if (e == hipSuccess) e = hipMalloc((void **)&h->U1, bytes);
+  if (e == hipSuccess && h->nd == 3) e = hipMalloc((void **)&h->acc, bytes);
   if (e == hipSuccess) e = hipMalloc((void **)&h->red, 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipHostMalloc((void **)&h->red_host, 4 * sizeof(unsigned long long), hipHostMallocDefault);
+  if (e == hipSuccess && cfg->resistive) {
+    std::vector<double> eh;
+    h->eta_max = eta_faces(h->cfg, eh);
+    e = hipMalloc((void **)&h->eta, eh.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(h->eta, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     omhd_destroy(h);
     return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
@@ -419,6 +661,8 @@ extern "C" int omhd_destroy(omhd_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->U) (void)hipFree(h->U);
   if (h->U1) (void)hipFree(h->U1);
+  if (h->acc) (void)hipFree(h->acc);
+  if (h->eta) (void)hipFree(h->eta);
   if (h->red) (void)hipFree(h->red);
   if (h->red_host) (void)hipHostFree(h->red_host);
   for (int i = 0; i < 3; i++)
@@ -438,29 +682,45 @@ extern "C" int omhd_state_layout(omhd_handle *h, double **dev_state, int64_t *pl
 
 extern "C" void *omhd_stream(omhd_handle *h) { return h ? (void *)h->stream : nullptr; }
 
+static size_t ncells(const omhd_handle *h) { return (size_t)h->cfg.nx * h->cfg.ny * h->cfg.nz; }
+static size_t nfortran(const omhd_handle *h) {
+  return (size_t)(h->cfg.nx + 2) * (h->cfg.ny + 2) * (h->nd == 3 ? h->cfg.nz + 2 : 1);
+}
+
+static int state_error(const int err[4]) {
+  const char *what = err[0] == OMHD_ERR_PRESSURE ? "negative pressure"
+                     : err[0] == OMHD_ERR_DENSITY ? "negative density"
+                                                  : "non-finite signal speed";
+  return set_error(err[0], what, err[1], err[2], err[3]);
+}
+
 static int cfl_pass(omhd_handle *h) {
-  const int nx = h->cfg.nx, ny = h->cfg.ny;
+  const omhd_config &c = h->cfg;
   k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
-  const long long n = (long long)nx * ny;
-  k_cfl2d<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->U, h->plane, h->pitch, nx, ny, h->red,
-                                                              (int *)(h->red + 1));
+  const size_t n = ncells(h);
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (h->nd == 3)
+    k_cfl<3><<<grid, 256, 0, h->stream>>>(h->U, h->plane, h->zstride, h->pitch, c.nx, c.ny, c.nz, h->red,
+                                          (int *)(h->red + 1));
+  else
+    k_cfl<2><<<grid, 256, 0, h->stream>>>(h->U, h->plane, h->zstride, h->pitch, c.nx, c.ny, c.nz, h->red,
+                                          (int *)(h->red + 1));
   OMHD_HIP(hipGetLastError());
-  int err[3];
+  int err[4];
   double vm;
   if (int rc = read_red(h, vm, err)) return rc;
-  if (err[0]) return set_error(err[0], "invalid state", err[1], err[2], 0);
+  if (err[0]) return state_error(err);
   h->vmax = vm;
   return OMHD_OK;
 }
 
 extern "C" int omhd_upload(omhd_handle *h, const double *U) {
   if (!h || !U) return set_error(OMHD_ERR_ARG, "null argument");
-  const int nx = h->cfg.nx, ny = h->cfg.ny;
-  const size_t n = (size_t)(nx + 2) * (ny + 2);
+  const omhd_config &c = h->cfg;
   Staged a;
-  if (int rc = a.init(U, 9 * n, true, false, h->stream)) return rc;
-  k_from_fortran<<<(unsigned)(((size_t)nx * ny + 255) / 256), 256, 0, h->stream>>>(a.dev, h->U, h->plane, h->pitch,
-                                                                                  nx, ny);
+  if (int rc = a.init(U, 9 * nfortran(h), true, false, h->stream)) return rc;
+  k_from_fortran<<<(unsigned)((ncells(h) + 255) / 256), 256, 0, h->stream>>>(a.dev, h->U, h->plane, h->zstride,
+                                                                             h->pitch, c.nx, c.ny, c.nz, h->nd);
   OMHD_HIP(hipGetLastError());
   a.release(h->stream);
   if (int rc = cfl_pass(h)) return rc;
@@ -470,15 +730,15 @@ extern "C" int omhd_upload(omhd_handle *h, const double *U) {
 
 extern "C" int omhd_download(omhd_handle *h, double *U, double *V) {
   if (!h || !U) return set_error(OMHD_ERR_ARG, "null argument");
-  const int nx = h->cfg.nx, ny = h->cfg.ny;
-  const size_t n = (size_t)(nx + 2) * (ny + 2);
+  const omhd_config &c = h->cfg;
+  const size_t n = nfortran(h);
   Staged a, b;
   if (int rc = a.init(U, 9 * n, false, true, h->stream)) return rc;
   if (V)
     if (int rc = b.init(V, 4 * n, false, true, h->stream)) return rc;
-  k_to_fortran<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->U, h->plane, h->pitch, nx, ny, h->cfg.bc_type[0],
-                                                                   h->cfg.bc_type[2], h->sxl, h->sxh, h->syl, h->syh,
-                                                                   a.dev, V ? b.dev : nullptr);
+  k_to_fortran<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->U, h->plane, h->zstride, h->pitch, c.nx, c.ny,
+                                                                   c.nz, h->nd, h->faces, a.dev,
+                                                                   V ? b.dev : nullptr);
   OMHD_HIP(hipGetLastError());
   if (int rc = a.finish(h->stream)) return rc;
   if (V)
@@ -489,65 +749,100 @@ extern "C" int omhd_download(omhd_handle *h, double *U, double *V) {
   return ok();
 }
 
+static Sweep base_sweep(omhd_handle *h, int pass) {
+  const omhd_config &c = h->cfg;
+  Sweep p{};
+  p.plane = h->plane;
+  p.zstride = h->zstride;
+  p.pitch = h->pitch;
+  p.nx = c.nx;
+  p.ny = c.ny;
+  p.nz = c.nz;
+  p.nstrip = (c.nx + STRIP - 1) / STRIP;
+  p.rows = h->rows;
+  const int nm = pass == PASS_Z ? c.nz : c.ny;
+  const int nslice = pass == PASS_2D ? 1 : pass == PASS_XY ? c.nz : c.ny;
+  p.nchunk = (nm + p.rows - 1) / p.rows;
+  p.ntiles = p.nstrip * p.nchunk * nslice;
+  for (int a = 0; a < 3; a++) {
+    p.bc[a] = h->faces.bc[a];
+    p.slo[a] = h->faces.slo[a];
+    p.shi[a] = h->faces.shi[a];
+  }
+  p.dx = c.dx;
+  p.eta = h->eta;
+  p.acc = h->acc;
+  p.vmax_bits = h->red;
+  p.err = (int *)(h->red + 1);
+  return p;
+}
+
+static unsigned grid_of(const Sweep &p) { return (unsigned)((p.ntiles + 7) / 8 * 8); }
+
 extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
   if (done) *done = 0;
   if (!h) return set_error(OMHD_ERR_ARG, "null handle");
   if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
   const omhd_config &c = h->cfg;
-  Sweep2D p{};
-  p.plane = h->plane;
-  p.pitch = h->pitch;
-  p.nx = c.nx;
-  p.ny = c.ny;
-  p.nstrip = (c.nx + STRIP - 1) / STRIP;
-  p.rows = h->rows;
-  const int nchunk = (c.ny + p.rows - 1) / p.rows;
-  p.ntiles = p.nstrip * nchunk;
-  p.bcx = c.bc_type[0];
-  p.bcy = c.bc_type[2];
-  p.sx_lo = h->sxl;
-  p.sx_hi = h->sxh;
-  p.sy_lo = h->syl;
-  p.sy_hi = h->syh;
-  p.vmax_bits = h->red;
-  p.err = (int *)(h->red + 1);
-  const unsigned grid = (unsigned)((p.ntiles + 7) / 8 * 8);
-  sweep_fn s1 = sweep_kernel(c.lm_type, 1, c.time_type, h->nt_store);
-  sweep_fn s2 = sweep_kernel(c.lm_type, 2, c.time_type, h->nt_store);
+  const bool three = h->nd == 3;
+  const int res = c.resistive ? 1 : 0;
+  Sweep pa = base_sweep(h, three ? PASS_XY : PASS_2D);
+  Sweep pz = three ? base_sweep(h, PASS_Z) : pa;
+  const march_fn a1 = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, 1, c.time_type);
+  const march_fn a2 = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, 2, c.time_type);
+  const march_fn z1 = three ? march_kernel(PASS_Z, res, c.lm_type, 1, c.time_type) : nullptr;
+  const march_fn z2 = three ? march_kernel(PASS_Z, res, c.lm_type, 2, c.time_type) : nullptr;
   const double cr = 0.2;  // glm_ss2.f90:18
   for (int n = 0; n < nsteps; n++) {
-    const double ch = h->vmax;                // set_dt returns vmax as ch (main.f90:80)
-    const double dt = c.cfl * c.dx / h->vmax;  // set_dt.f90:52
+    const double ch = h->vmax;           // set_dt returns vmax as ch (main.f90:80)
+    double dt = c.cfl * c.dx / h->vmax;  // set_dt.f90:52
     if (!(dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, dt);  // set_dt.f90:55-65
-    p.ch = ch;
-    p.ch2 = ch * ch;
-    p.glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
+    if (res) {  // set_dt_resistive (flux_resistive.cuf:2-9) with the configured eta
+      const double dr = 0.5 * c.cfl * (c.dx * c.dx) / h->eta_max;
+      dt = dt < dr ? dt : dr;
+    }
+    const double glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
+    for (Sweep *q : {&pa, &pz}) {
+      q->ch = ch;
+      q->ch2 = ch * ch;
+      q->glm_f = glm_f;
+    }
     k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[0], h->stream));
     // stage 1: U* = U + dtx dF   (rk_tvd21: dt/dx, rk_std21: 0.5*dt/dx)
-    p.us = h->U;
-    p.u0 = h->U;
-    p.out = h->U1;
-    p.dtx = (c.time_type == 0) ? dt / c.dx : 0.5 * dt / c.dx;
-    hipLaunchKernelGGL(s1, dim3(grid), dim3(64), 0, h->stream, p);
+    const double dtx1 = (c.time_type == 0) ? dt / c.dx : 0.5 * dt / c.dx;
+    for (Sweep *q : {&pa, &pz}) {
+      q->us = h->U;
+      q->u0 = h->U;
+      q->out = h->U1;
+      q->dtx = dtx1;
+    }
+    hipLaunchKernelGGL(a1, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
     OMHD_HIP(hipGetLastError());
+    if (three) {
+      hipLaunchKernelGGL(z1, dim3(grid_of(pz)), dim3(64), 0, h->stream, pz);
+      OMHD_HIP(hipGetLastError());
+    }
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[1], h->stream));
     // stage 2: U^{n+1} in place over U^n (each cell's U^n is read only by the
-    // wave that overwrites it, just before it does)
-    p.us = h->U1;
-    p.u0 = h->U;
-    p.out = h->U;
-    p.dtx = dt / c.dx;
-    hipLaunchKernelGGL(s2, dim3(grid), dim3(64), 0, h->stream, p);
+    // wavefront that overwrites it, just before it does)
+    for (Sweep *q : {&pa, &pz}) {
+      q->us = h->U1;
+      q->u0 = h->U;
+      q->out = h->U;
+      q->dtx = dt / c.dx;
+    }
+    hipLaunchKernelGGL(a2, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
     OMHD_HIP(hipGetLastError());
+    if (three) {
+      hipLaunchKernelGGL(z2, dim3(grid_of(pz)), dim3(64), 0, h->stream, pz);
+      OMHD_HIP(hipGetLastError());
+    }
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[2], h->stream));
-    int err[3];
+    int err[4];
     double vm;
     if (int rc = read_red(h, vm, err)) return rc;
-    if (err[0]) return set_error(err[0], err[0] == OMHD_ERR_PRESSURE ? "negative pressure"
-                                         : err[0] == OMHD_ERR_DENSITY ? "negative density"
-                                                                      : "non-finite signal speed",
-                                 err[1], err[2], 0);
+    if (err[0]) return state_error(err);
     h->vmax = vm;
     if (h->timing) {  // read_red synchronised the stream: both events are complete
       float a = 0.f, b = 0.f;
@@ -582,5 +877,426 @@ extern "C" int omhd_get_timing(omhd_handle *h, double *stage_ms, int64_t *launch
     stage_ms[i] = h->stage_ms[i];
     launches[i] = h->stage_launches[i];
   }
+  return ok();
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble of independent 2D grids (batched step).
+//
+// Every member keeps its own SoA state; one tile table lists (member, tile)
+// pairs for all members, and each RK stage is one persistent launch over it.
+// Each member has its own dt: either computed on the host from its reduced CFL
+// speed exactly as omhd_step does (host dt mode: every member is bitwise equal
+// to its solo run), or on the device (device dt mode: no host round trip, so
+// whole steps are captured in one hipGraph and replayed).
+// ---------------------------------------------------------------------------
+struct omhd_batch {
+  int n = 0;
+  std::vector<omhd_config> cfg;
+  std::vector<Sweep> mem;  // host copy of the member descriptors
+  Sweep *mem_d = nullptr;
+  int2 *tiles_d = nullptr;
+  int ntiles = 0;
+  double *arena = nullptr;  // all U and U1 buffers
+  size_t arena_doubles = 0;
+  double *backup = nullptr;  // autotuner: state saved around trials
+  unsigned long long *vmax_d = nullptr;
+  int *err_d = nullptr;
+  DevPar *par_d = nullptr;
+  double *dx_d = nullptr, *cfl_d = nullptr, *tsum_d = nullptr;
+  int *tt_d = nullptr;
+  std::vector<DevPar> par_h;
+  std::vector<unsigned long long> vmax_h;
+  std::vector<double> vmax;  // CFL speed of each member's current state
+  std::vector<double> t;
+  hipStream_t stream = nullptr;
+  int rows = 24, grid = 0;
+  bool device_dt = false, use_graph = false, have_state = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int lim = 1, tt = 0;
+};
+
+static int batch_tiles(omhd_batch *b) {
+  std::vector<int2> tl;
+  for (int i = 0; i < b->n; i++) {
+    Sweep &p = b->mem[i];
+    p.rows = b->rows;
+    p.nchunk = (p.ny + p.rows - 1) / p.rows;
+    p.ntiles = p.nstrip * p.nchunk;
+    for (int t = 0; t < p.ntiles; t++) tl.push_back(make_int2(i, t));
+  }
+  b->ntiles = (int)tl.size();
+  if (b->tiles_d) OMHD_HIP(hipFree(b->tiles_d));
+  OMHD_HIP(hipMalloc((void **)&b->tiles_d, tl.size() * sizeof(int2)));
+  OMHD_HIP(hipMemcpy(b->tiles_d, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice));
+  OMHD_HIP(hipMemcpy(b->mem_d, b->mem.data(), b->n * sizeof(Sweep), hipMemcpyHostToDevice));
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return OMHD_OK;
+}
+
+typedef void (*ens_fn)(const Sweep *, const int2 *, int);
+
+static ens_fn ens_kernel(int lim, int stage, int tt) {
+#define OMHD_ENS(L)                                                                                 \
+  if (lim == L) {                                                                                   \
+    if (stage == 1) return tt == 0 ? k_march_ens<L, 1, 0> : k_march_ens<L, 1, 1>;                  \
+    return tt == 0 ? k_march_ens<L, 2, 0> : k_march_ens<L, 2, 1>;                                  \
+  }
+  OMHD_ENS(0) OMHD_ENS(1) OMHD_ENS(2) OMHD_ENS(3) OMHD_ENS(4)
+#undef OMHD_ENS
+  return nullptr;
+}
+
+extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch **out) {
+  if (!cfgs || !out || n < 1) return set_error(OMHD_ERR_ARG, "need n >= 1 configurations");
+  for (int i = 0; i < n; i++) {
+    const omhd_config &c = cfgs[i];
+    if (c.ndim != 2) return set_error(OMHD_ERR_ARG, "ensemble members are 2D grids", i);
+    if (c.nx < 2 || c.ny < 2) return set_error(OMHD_ERR_EXTENT, "need nx, ny >= 2", c.nx, c.ny, i);
+    if (c.flux_type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", i);
+    if (c.lm_type != cfgs[0].lm_type || c.time_type != cfgs[0].time_type)
+      return set_error(OMHD_ERR_ARG, "members share one limiter and one time type (one kernel per stage)", i);
+    if (c.lm_type < 0 || c.lm_type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", c.lm_type);
+    if (c.resistive) return set_error(OMHD_ERR_ARG, "ensemble members are ideal MHD", i);
+    for (int a = 0; a < 2; a++)
+      if (c.bc_type[2 * a] != c.bc_type[2 * a + 1])
+        return set_error(OMHD_ERR_ARG, "both faces of an axis share one boundary type", i);
+  }
+  omhd_batch *b = new omhd_batch();
+  b->n = n;
+  b->cfg.assign(cfgs, cfgs + n);
+  b->lim = cfgs[0].lm_type;
+  b->tt = cfgs[0].time_type;
+  b->mem.resize(n);
+  b->par_h.resize(n);
+  b->vmax_h.resize(n);
+  b->vmax.assign(n, -1.0);
+  b->t.assign(n, 0.0);
+  std::vector<size_t> off(n);
+  size_t total = 0;
+  for (int i = 0; i < n; i++) {
+    const omhd_config &c = cfgs[i];
+    Sweep &p = b->mem[i];
+    p = Sweep{};
+    p.nx = c.nx;
+    p.ny = c.ny;
+    p.nz = 1;
+    p.pitch = (c.nx + 7) & ~7;
+    p.zstride = (long long)p.pitch * c.ny;
+    p.plane = (p.zstride + 63) & ~63LL;
+    p.nstrip = (c.nx + STRIP - 1) / STRIP;
+    for (int a = 0; a < 3; a++) {
+      p.bc[a] = c.bc_type[2 * a];
+      p.slo[a] = sign_mask(c.bc_sign + 18 * a);
+      p.shi[a] = sign_mask(c.bc_sign + 18 * a + 9);
+    }
+    p.dx = c.dx;
+    off[i] = total;
+    total += 2 * (size_t)p.plane * NV;
+  }
+  b->arena_doubles = total;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->arena, total * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->mem_d, n * sizeof(Sweep));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->vmax_d, n * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->err_d, 4 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->par_d, n * sizeof(DevPar));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->dx_d, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->cfl_d, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->tsum_d, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->tt_d, n * sizeof(int));
+  if (e != hipSuccess) {
+    omhd_batch_destroy(b);
+    return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
+  }
+  std::vector<double> dx(n), cfl(n);
+  std::vector<int> tt(n);
+  for (int i = 0; i < n; i++) {
+    Sweep &p = b->mem[i];
+    p.U = b->arena + off[i];
+    p.U1 = p.U + (size_t)p.plane * NV;
+    p.vmax_bits = b->vmax_d + i;
+    p.err = b->err_d;
+    p.dpar = (const double *)(b->par_d + i);
+    dx[i] = cfgs[i].dx;
+    cfl[i] = cfgs[i].cfl;
+    tt[i] = cfgs[i].time_type;
+  }
+  OMHD_HIP(hipMemcpy(b->dx_d, dx.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  OMHD_HIP(hipMemcpy(b->cfl_d, cfl.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  OMHD_HIP(hipMemcpy(b->tt_d, tt.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  OMHD_HIP(hipMemset(b->err_d, 0, 4 * sizeof(int)));
+  OMHD_HIP(hipMemset(b->tsum_d, 0, n * sizeof(double)));
+  if (int rc = batch_tiles(b)) return rc;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = 0;  // 0: one workgroup per tile; >0: persistent grid of that many workgroups
+  *out = b;
+  return ok();
+}
+
+extern "C" int omhd_batch_destroy(omhd_batch *b) {
+  if (!b) return ok();
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *q : {(void *)b->arena, (void *)b->backup, (void *)b->mem_d, (void *)b->tiles_d, (void *)b->vmax_d,
+                  (void *)b->err_d, (void *)b->par_d, (void *)b->dx_d, (void *)b->cfl_d, (void *)b->tsum_d,
+                  (void *)b->tt_d})
+    if (q) (void)hipFree(q);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return ok();
+}
+
+static int batch_check_err(omhd_batch *b) {
+  int err[4];
+  OMHD_HIP(hipMemcpyAsync(err, b->err_d, sizeof err, hipMemcpyDeviceToHost, b->stream));
+  OMHD_HIP(hipStreamSynchronize(b->stream));
+  if (err[0]) return state_error(err);
+  return OMHD_OK;
+}
+
+extern "C" int omhd_batch_upload(omhd_batch *b, int32_t i, const double *U) {
+  if (!b || !U || i < 0 || i >= b->n) return set_error(OMHD_ERR_ARG, "bad member or null array");
+  const Sweep &p = b->mem[i];
+  const size_t nf = (size_t)(p.nx + 2) * (p.ny + 2);
+  Staged a;
+  if (int rc = a.init(U, 9 * nf, true, false, b->stream)) return rc;
+  k_from_fortran<<<(unsigned)(((size_t)p.nx * p.ny + 255) / 256), 256, 0, b->stream>>>(a.dev, p.U, p.plane, p.zstride,
+                                                                                        p.pitch, p.nx, p.ny, 1, 2);
+  OMHD_HIP(hipGetLastError());
+  a.release(b->stream);
+  // first CFL speed of this member (main.f90:40)
+  OMHD_HIP(hipMemsetAsync(b->err_d, 0, 4 * sizeof(int), b->stream));
+  OMHD_HIP(hipMemsetAsync(b->vmax_d + i, 0, sizeof(unsigned long long), b->stream));
+  k_cfl<2><<<(unsigned)(((size_t)p.nx * p.ny + 255) / 256), 256, 0, b->stream>>>(p.U, p.plane, p.zstride, p.pitch,
+                                                                                  p.nx, p.ny, 1, b->vmax_d + i,
+                                                                                  b->err_d);
+  OMHD_HIP(hipGetLastError());
+  unsigned long long v;
+  OMHD_HIP(hipMemcpyAsync(&v, b->vmax_d + i, sizeof v, hipMemcpyDeviceToHost, b->stream));
+  if (int rc = batch_check_err(b)) return rc;
+  memcpy(&b->vmax[i], &v, sizeof(double));
+  b->t[i] = 0.0;
+  b->have_state = true;
+  return ok();
+}
+
+extern "C" int omhd_batch_download(omhd_batch *b, int32_t i, double *U, double *V) {
+  if (!b || !U || i < 0 || i >= b->n) return set_error(OMHD_ERR_ARG, "bad member or null array");
+  const Sweep &p = b->mem[i];
+  const size_t n = (size_t)(p.nx + 2) * (p.ny + 2);
+  Faces f{};
+  for (int a = 0; a < 3; a++) {
+    f.bc[a] = p.bc[a];
+    f.slo[a] = p.slo[a];
+    f.shi[a] = p.shi[a];
+  }
+  Staged a, bb;
+  if (int rc = a.init(U, 9 * n, false, true, b->stream)) return rc;
+  if (V)
+    if (int rc = bb.init(V, 4 * n, false, true, b->stream)) return rc;
+  k_to_fortran<<<(unsigned)((n + 255) / 256), 256, 0, b->stream>>>(p.U, p.plane, p.zstride, p.pitch, p.nx, p.ny, 1, 2,
+                                                                   f, a.dev, V ? bb.dev : nullptr);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = a.finish(b->stream)) return rc;
+  if (V)
+    if (int rc = bb.finish(b->stream)) return rc;
+  a.release(b->stream);
+  bb.release(b->stream);
+  OMHD_HIP(hipStreamSynchronize(b->stream));
+  return ok();
+}
+
+// dt mode and graph capture.  Entering device dt mode seeds the device-side
+// parameters from the current speeds.
+extern "C" int omhd_batch_set_mode(omhd_batch *b, int32_t device_dt, int32_t use_graph) {
+  if (!b) return set_error(OMHD_ERR_ARG, "null batch");
+  if (use_graph && !device_dt) return set_error(OMHD_ERR_ARG, "graph capture needs device dt mode");
+  if (device_dt && !b->device_dt) {
+    std::vector<unsigned long long> bits(b->n);
+    for (int i = 0; i < b->n; i++) memcpy(&bits[i], &b->vmax[i], sizeof(double));
+    OMHD_HIP(hipMemcpy(b->vmax_d, bits.data(), b->n * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    OMHD_HIP(hipMemcpy(b->tsum_d, b->t.data(), b->n * sizeof(double), hipMemcpyHostToDevice));
+    k_device_dt<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->par_d, b->vmax_d, b->dx_d, b->cfl_d, b->tt_d, nullptr,
+                                                        b->err_d, b->n);
+    OMHD_HIP(hipGetLastError());
+    if (int rc = batch_check_err(b)) return rc;
+  }
+  if (!device_dt && b->device_dt) {  // back to host dt: fetch the speeds of the current states
+    std::vector<DevPar> par(b->n);
+    OMHD_HIP(hipMemcpy(par.data(), b->par_d, b->n * sizeof(DevPar), hipMemcpyDeviceToHost));
+    OMHD_HIP(hipMemcpy(b->t.data(), b->tsum_d, b->n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n; i++) b->vmax[i] = par[i].ch;
+  }
+  b->device_dt = device_dt != 0;
+  b->use_graph = use_graph != 0;
+  return ok();
+}
+
+static int batch_launch_stages(omhd_batch *b) {
+  const unsigned grid = b->grid > 0 ? (unsigned)b->grid : (unsigned)b->ntiles;
+  hipLaunchKernelGGL(ens_kernel(b->lim, 1, b->tt), dim3(grid), dim3(64), 0, b->stream, b->mem_d, b->tiles_d,
+                     b->ntiles);
+  OMHD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ens_kernel(b->lim, 2, b->tt), dim3(grid), dim3(64), 0, b->stream, b->mem_d, b->tiles_d,
+                     b->ntiles);
+  OMHD_HIP(hipGetLastError());
+  return OMHD_OK;
+}
+
+__global__ void k_accumulate_t(double *tsum, const DevPar *par, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tsum[i] += par[i].dt;
+}
+
+static int batch_device_step(omhd_batch *b) {
+  // the parameters in par_d are those of this step; the dt kernel then
+  // prepares the next step's from the speeds the stage-2 pass reduced
+  k_accumulate_t<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->tsum_d, b->par_d, b->n);
+  if (int rc = batch_launch_stages(b)) return rc;
+  k_device_dt<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->par_d, b->vmax_d, b->dx_d, b->cfl_d, b->tt_d, nullptr,
+                                                      b->err_d, b->n);
+  OMHD_HIP(hipGetLastError());
+  return OMHD_OK;
+}
+
+extern "C" int omhd_batch_step(omhd_batch *b, int32_t nsteps, double *dt_trace) {
+  if (!b) return set_error(OMHD_ERR_ARG, "null batch");
+  if (!b->have_state) return set_error(OMHD_ERR_ARG, "upload the members first");
+  const double cr = 0.2;  // glm_ss2.f90:18
+  if (b->device_dt) {
+    if (b->use_graph && !b->gexec) {
+      OMHD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_device_step(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
+      b->graph = g;
+      OMHD_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; s++) {
+      if (b->use_graph) {
+        OMHD_HIP(hipGraphLaunch(b->gexec, b->stream));
+      } else if (int rc = batch_device_step(b)) {
+        return rc;
+      }
+      if (dt_trace) {  // per-step trace costs a round trip
+        OMHD_HIP(hipMemcpyAsync(b->par_h.data(), b->par_d, b->n * sizeof(DevPar), hipMemcpyDeviceToHost, b->stream));
+        if (int rc = batch_check_err(b)) return rc;
+        // par_d now holds the next step's parameters; the dt just taken is the
+        // difference of the accumulated times
+        std::vector<double> tnew(b->n);
+        OMHD_HIP(hipMemcpy(tnew.data(), b->tsum_d, b->n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < b->n; i++) dt_trace[(size_t)s * b->n + i] = tnew[i] - b->t[i];
+        b->t = tnew;
+      }
+    }
+    if (int rc = batch_check_err(b)) return rc;
+    std::vector<DevPar> par(b->n);
+    OMHD_HIP(hipMemcpy(par.data(), b->par_d, b->n * sizeof(DevPar), hipMemcpyDeviceToHost));
+    OMHD_HIP(hipMemcpy(b->t.data(), b->tsum_d, b->n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n; i++) b->vmax[i] = par[i].ch;
+    return ok();
+  }
+  for (int s = 0; s < nsteps; s++) {
+    for (int i = 0; i < b->n; i++) {  // exactly omhd_step's host arithmetic, per member
+      const omhd_config &c = b->cfg[i];
+      const double ch = b->vmax[i];
+      const double dt = c.cfl * c.dx / ch;
+      if (!(dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", i, 0, 0, dt);
+      DevPar &q = b->par_h[i];
+      q.ch = ch;
+      q.ch2 = ch * ch;
+      q.dtx1 = (c.time_type == 0) ? dt / c.dx : 0.5 * dt / c.dx;
+      q.dtx2 = dt / c.dx;
+      q.glm_f = exp(-0.5 * dt * ch / cr);
+      q.dt = dt;
+    }
+    OMHD_HIP(hipMemcpyAsync(b->par_d, b->par_h.data(), b->n * sizeof(DevPar), hipMemcpyHostToDevice, b->stream));
+    OMHD_HIP(hipMemsetAsync(b->vmax_d, 0, b->n * sizeof(unsigned long long), b->stream));
+    if (int rc = batch_launch_stages(b)) return rc;
+    OMHD_HIP(hipMemcpyAsync(b->vmax_h.data(), b->vmax_d, b->n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            b->stream));
+    if (int rc = batch_check_err(b)) return rc;
+    for (int i = 0; i < b->n; i++) {
+      memcpy(&b->vmax[i], &b->vmax_h[i], sizeof(double));
+      b->t[i] += b->par_h[i].dt;
+      if (dt_trace) dt_trace[(size_t)s * b->n + i] = b->par_h[i].dt;
+    }
+  }
+  return ok();
+}
+
+extern "C" int omhd_batch_time(omhd_batch *b, double *t) {
+  if (!b || !t) return set_error(OMHD_ERR_ARG, "null argument");
+  for (int i = 0; i < b->n; i++) t[i] = b->t[i];
+  return ok();
+}
+
+// Tile-shape / occupancy autotuner: times `trial_steps` steps for each
+// candidate (rows per tile x launch shape), restoring the ensemble state after
+// every trial, and keeps the fastest.  Returns the choice in rows/grid.
+extern "C" int omhd_batch_autotune(omhd_batch *b, int32_t trial_steps, int32_t *rows_out, int32_t *grid_out) {
+  if (!b || trial_steps < 1) return set_error(OMHD_ERR_ARG, "bad arguments");
+  if (!b->have_state) return set_error(OMHD_ERR_ARG, "upload the members first");
+  if (!b->backup) OMHD_HIP(hipMalloc((void **)&b->backup, b->arena_doubles * sizeof(double)));
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int cand_rows[] = {12, 16, 24, 32, 48};
+  const int cand_grid[] = {0, 8 * cus, 16 * cus};  // one group per tile; persistent at 2 or 4 groups per SIMD
+  const std::vector<double> vmax0 = b->vmax, t0 = b->t;
+  const bool dmode = b->device_dt, gmode = b->use_graph;
+  hipEvent_t e0, e1;
+  OMHD_HIP(hipEventCreate(&e0));
+  OMHD_HIP(hipEventCreate(&e1));
+  OMHD_HIP(hipMemcpy(b->backup, b->arena, b->arena_doubles * sizeof(double), hipMemcpyDeviceToDevice));
+  double best = 1e300;
+  int best_rows = b->rows, best_grid = b->grid;
+  for (int r : cand_rows)
+    for (int g : cand_grid) {
+      b->rows = r;
+      b->grid = g;
+      if (int rc = batch_tiles(b)) return rc;
+      if (g > b->ntiles) continue;
+      OMHD_HIP(hipEventRecord(e0, b->stream));
+      if (int rc = omhd_batch_step(b, trial_steps, nullptr)) return rc;
+      OMHD_HIP(hipEventRecord(e1, b->stream));
+      OMHD_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      OMHD_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_rows = r;
+        best_grid = g;
+      }
+      // restore the state the trial advanced
+      OMHD_HIP(hipMemcpy(b->arena, b->backup, b->arena_doubles * sizeof(double), hipMemcpyDeviceToDevice));
+      b->vmax = vmax0;
+      b->t = t0;
+      if (dmode) {
+        b->device_dt = false;
+        if (int rc = omhd_batch_set_mode(b, 1, gmode)) return rc;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->rows = best_rows;
+  b->grid = best_grid;
+  if (int rc = batch_tiles(b)) return rc;
+  if (rows_out) *rows_out = best_rows;
+  if (grid_out) *grid_out = best_grid;
   return ok();
 }

@@ -78,7 +78,8 @@ program main2d_resident
      cfg%bc_sign(27 + 2) = -1.d0; cfg%bc_sign(27 + 7) = -1.d0
   endif
   cfg%rows_per_wave = 0
-  cfg%store_hint = 0
+  cfg%resistive = 0
+  cfg%x0 = 0.d0; cfg%y0 = 0.d0; cfg%eta0 = 0.d0; cfg%eta1 = 0.d0; cfg%eta_w = 1.d0
   cfg%reserved = 0
 
   rc = omhd_create(cfg, h)

@@ -25,8 +25,9 @@ module omhd_iface
      integer(c_int32_t) :: lm_type, flux_type, time_type
      integer(c_int32_t) :: bc_type(6)
      real(c_double) :: bc_sign(54)
-     integer(c_int32_t) :: rows_per_wave, store_hint
-     integer(c_int32_t) :: reserved(6)
+     integer(c_int32_t) :: rows_per_wave, resistive
+     real(c_double) :: x0, y0, eta0, eta1, eta_w
+     integer(c_int32_t) :: reserved(8)
   end type omhd_config
 
   interface

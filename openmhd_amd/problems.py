@@ -107,3 +107,61 @@ def kelvin_helmholtz(nx: int, ny: int, seed: int = 1, delta: float = 1.0, lx: fl
     U = prim_to_cons_fortran(one, vx, vy, z, p * one, bx * one, z, bz * one, z)
     del X
     return U, dx
+
+
+def harris_sheet_3d(nx: int, ny: int, nz: int, dx: float, seed: int = 5, beta_b: float = 0.2, rho0: float = 0.2,
+                    lam: float = 1.0, noise: float = 1.0e-3):
+    """3D Harris current sheet for the resistive reconnection configuration:
+    Bx = tanh(y/lam), rho = rho0 + cosh(y/lam)^-2 / beta_b, isothermal
+    p = beta_b * rho / 2 (p + B^2/2 uniform), seeded Gaussian noise of
+    amplitude `noise` in all three velocity components.  The domain is
+    x in [-nx dx/2, nx dx/2), y in [-ny dx/2, ny dx/2], z in [0, nz dx) (the
+    sheet and the resistivity peak at x = y = 0).  Returns (U, origin) with
+    U(ix,jx,kx,9) Fortran-ordered and origin = (x0, y0)."""
+    ix, jx, kx = nx + 2, ny + 2, nz + 2
+    x0, y0 = -0.5 * nx * dx, -0.5 * ny * dx
+    y = y0 + (np.arange(jx) - 0.5) * dx
+    Y = np.broadcast_to(y[None, :, None], (ix, jx, kx))
+    I = np.broadcast_to(np.arange(ix)[:, None, None], (ix, jx, kx))
+    J = np.broadcast_to(np.arange(jx)[None, :, None], (ix, jx, kx))
+    K = np.broadcast_to(np.arange(kx)[None, None, :], (ix, jx, kx))
+    ro = rho0 + np.cosh(Y / lam) ** -2 / beta_b
+    p = 0.5 * beta_b * ro
+    bx = np.tanh(Y / lam)
+    key = (K * jx + J).astype(np.uint64)
+    vx = noise * counter_normal(seed, I, key)
+    vy = noise * counter_normal(seed + 1, I, key)
+    vz = noise * counter_normal(seed + 2, I, key)
+    z = np.zeros((ix, jx, kx))
+    f1 = 1.0 / (GAMMA - 1)
+    U = np.empty((ix, jx, kx, 9), order="F")
+    U[..., 0] = ro * vx
+    U[..., 1] = ro * vy
+    U[..., 2] = ro * vz
+    U[..., 3] = 0.5 * (ro * (vx * vx + vy * vy + vz * vz) + bx * bx) + f1 * p
+    U[..., 4] = ro
+    U[..., 5] = bx
+    U[..., 6] = z
+    U[..., 7] = z
+    U[..., 8] = z
+    return U, (x0, y0)
+
+
+def kh_ensemble(n: int, seed: int = 2026, smin: int = 64, smax: int = 1024):
+    """Parameters of an ensemble of independent KH runs drawn from one master
+    seed: grid sides nx, ny log-uniform in [smin, smax] (independently), shear
+    speed U0 ~ U[0.2, 1.0], beta ~ U[0.5, 5], Bx ~ U[0, 0.3], and a noise seed
+    per member.  Returns a list of dicts (nx, ny, u0, beta, bx, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(smin), np.log(smax))))) for _ in range(2))
+        out.append(dict(nx=nx, ny=ny, u0=float(rng.uniform(0.2, 1.0)), beta=float(rng.uniform(0.5, 5.0)),
+                        bx=float(rng.uniform(0.0, 0.3)), seed=int(rng.integers(1, 2**31 - 1))))
+    return out
+
+
+def kh_member(p: dict):
+    """Initial state of one ensemble member: the KH layer of kelvin_helmholtz
+    on an nx x ny grid of the member's parameters (Lx = 40, y extent ny*dx)."""
+    return kelvin_helmholtz(p["nx"], p["ny"], seed=p["seed"], u0=p["u0"], beta=p["beta"], bx=p["bx"])

@@ -13,29 +13,48 @@ from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, Config, check, lib
 from .api import KH_WALL_SIGN, _ptr
 
 
-class Solver2D:
-    """A 2D grid of nx*ny interior cells advanced on the GPU.
+def make_config(nx, ny, dx, nz=0, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic", "periodic"),
+                wall_sign=KH_WALL_SIGN, rows_per_wave=0, resistivity=None, origin=(0.0, 0.0)):
+    """omhd_config for a grid (see Solver)."""
+    cfg = Config()
+    cfg.ndim = 3 if nz else 2
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, max(nz, 1)
+    cfg.dx, cfg.cfl = dx, cfl
+    cfg.lm_type, cfg.flux_type, cfg.time_type = lm_type, flux_type, time_type
+    per_axis = len(wall_sign) == 3 and hasattr(wall_sign[0], "__len__")
+    for a in range(3):
+        kind = bc[a] if a < len(bc) else "periodic"
+        sg = wall_sign[a] if per_axis else wall_sign
+        for s in range(2):
+            f = 2 * a + s
+            cfg.bc_type[f] = OMHD_BC_MIRROR if kind == "wall" else OMHD_BC_PERIODIC
+            for m in range(9):
+                cfg.bc_sign[9 * f + m] = sg[m] if kind == "wall" else 1.0
+    cfg.rows_per_wave = rows_per_wave
+    cfg.x0, cfg.y0 = origin
+    if resistivity is not None:
+        cfg.resistive = 1
+        cfg.eta0, cfg.eta1, cfg.eta_w = resistivity
+    return cfg
+
+
+class Solver:
+    """A grid of nx*ny (2D) or nx*ny*nz (3D, nz > 0) interior cells advanced
+    on the GPU.
 
     bc: per axis ('periodic' | 'wall'); wall_sign: 9 signs (+1/-1) applied to
-    the mirrored ghost (default: the KH walls of 2D_KH/bc.f90).
+    the mirrored ghost (default: the KH walls of 2D_KH/bc.f90), or a list of
+    one 9-vector per axis.  resistivity: (eta0, eta1, w) for the 3D resistive
+    step with eta = eta0 + eta1 exp(-(x^2+y^2)/w^2) at the face centres;
+    origin: (x0, y0), the lower corner of the domain.
     """
 
-    def __init__(self, nx, ny, dx, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic", "periodic"),
-                 wall_sign=KH_WALL_SIGN, rows_per_wave=0, store_hint=0):
-        self.nx, self.ny, self.dx = nx, ny, dx
-        cfg = Config()
-        cfg.ndim, cfg.nx, cfg.ny, cfg.nz = 2, nx, ny, 1
-        cfg.dx, cfg.cfl = dx, cfl
-        cfg.lm_type, cfg.flux_type, cfg.time_type = lm_type, flux_type, time_type
-        for a in range(3):
-            kind = bc[a] if a < len(bc) else "periodic"
-            for s in range(2):
-                f = 2 * a + s
-                cfg.bc_type[f] = OMHD_BC_MIRROR if kind == "wall" else OMHD_BC_PERIODIC
-                for m in range(9):
-                    cfg.bc_sign[9 * f + m] = wall_sign[m] if kind == "wall" else 1.0
-        cfg.rows_per_wave = rows_per_wave
-        cfg.store_hint = store_hint
+    def __init__(self, nx, ny, dx, nz=0, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic", "periodic"),
+                 wall_sign=KH_WALL_SIGN, rows_per_wave=0, resistivity=None, origin=(0.0, 0.0)):
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.ndim = 3 if nz else 2
+        cfg = make_config(nx, ny, dx, nz, cfl, lm_type, flux_type, time_type, bc, wall_sign, rows_per_wave,
+                          resistivity, origin)
         self.cfg = cfg
         h = C.c_void_p()
         check(lib.omhd_create(C.byref(cfg), C.byref(h)), "omhd_create")
@@ -43,14 +62,18 @@ class Solver2D:
         self.t = 0.0
 
     # -- state transfer -----------------------------------------------------
+    def _shape(self):
+        s = (self.nx + 2, self.ny + 2)
+        return s + (self.nz + 2,) if self.ndim == 3 else s
+
     def upload(self, U):
-        """U: (nx+2, ny+2, 9) Fortran-ordered (ghosts ignored), host or device."""
-        check(lib.omhd_upload(self.h, _ptr(U, 9 * (self.nx + 2) * (self.ny + 2))), "omhd_upload")
+        """U: (nx+2, ny+2[, nz+2], 9) Fortran-ordered (ghosts ignored), host or device."""
+        check(lib.omhd_upload(self.h, _ptr(U, 9 * int(np.prod(self._shape())))), "omhd_upload")
 
     def download(self, with_v=False):
-        ix, jx = self.nx + 2, self.ny + 2
-        U = np.zeros((ix, jx, 9), order="F")
-        V = np.zeros((ix, jx, 4), order="F") if with_v else None
+        shp = self._shape()
+        U = np.zeros(shp + (9,), order="F")
+        V = np.zeros(shp + (4,), order="F") if with_v else None
         check(lib.omhd_download(self.h, _ptr(U), _ptr(V) if with_v else None), "omhd_download")
         return (U, V) if with_v else U
 
@@ -87,6 +110,63 @@ class Solver2D:
     def close(self):
         if getattr(self, "h", None):
             lib.omhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+Solver2D = Solver  # the 2D form of the same class (nz = 0)
+
+
+class Ensemble:
+    """n independent 2D grids advanced together (omhd_batch_*): one persistent
+    launch per RK stage over a tile table of all members, each member with its
+    own CFL dt.  configs: a list of omhd_config (make_config(...)); members share
+    the limiter and time type.  device_dt=False: every member is bitwise equal
+    to its solo Solver run; device_dt=True (+ graph=True): dt computed on the
+    GPU, whole steps captured in a hipGraph."""
+
+    def __init__(self, configs):
+        self.n = len(configs)
+        arr = (Config * self.n)(*configs)
+        self.shapes = [(c.nx + 2, c.ny + 2) for c in configs]
+        h = C.c_void_p()
+        check(lib.omhd_batch_create(arr, self.n, C.byref(h)), "omhd_batch_create")
+        self.h = h
+
+    def upload(self, i, U):
+        ix, jx = self.shapes[i]
+        check(lib.omhd_batch_upload(self.h, i, _ptr(U, 9 * ix * jx)), "omhd_batch_upload")
+
+    def download(self, i):
+        ix, jx = self.shapes[i]
+        U = np.zeros((ix, jx, 9), order="F")
+        check(lib.omhd_batch_download(self.h, i, _ptr(U), None), "omhd_batch_download")
+        return U
+
+    def set_mode(self, device_dt=False, graph=False):
+        check(lib.omhd_batch_set_mode(self.h, 1 if device_dt else 0, 1 if graph else 0), "omhd_batch_set_mode")
+
+    def step(self, nsteps=1, trace=False):
+        """Returns the (nsteps, n) dt trace if trace."""
+        tr = np.zeros((nsteps, self.n)) if trace else None
+        check(lib.omhd_batch_step(self.h, nsteps, tr.ctypes.data_as(DP) if trace else None), "omhd_batch_step")
+        return tr
+
+    def times(self):
+        t = np.zeros(self.n)
+        check(lib.omhd_batch_time(self.h, t.ctypes.data_as(DP)), "omhd_batch_time")
+        return t
+
+    def autotune(self, trial_steps=3):
+        r, g = I32(), I32()
+        check(lib.omhd_batch_autotune(self.h, trial_steps, C.byref(r), C.byref(g)), "omhd_batch_autotune")
+        return r.value, g.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.omhd_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -567,6 +567,89 @@ void or_bc_face(double *VL, double *VR, int ix, int jx, int kx, int dir, int bct
 }
 
 /* ------------------------------------------------------------------------
+ * Resistive flux, 3D_reconnection_gpu/flux_resistive.cuf:86-135 (arithmetic
+ * of the reference's only 3D form; the 2D CPU form folds eta/dx into J and
+ * orders operations differently, 2D_reconnection/flux_resistive.f90:74-84).
+ * Adds eta*J terms to F(en), F(bt1), F(bt2) at the interfaces of the
+ * reference's loop range.  eta is given per face as an (ix,jx) array
+ * (eta(x,y) at the face centre; it is uniform in z), computed on the host.
+ * ---------------------------------------------------------------------- */
+void or_flux_resistive(double *F, const double *U, const double *eta_face, double dx, int ix, int jx, int kx, int dir)
+{
+  int is = 2, ie = ix - 1, js = 2, je = jx - 1, ks = 2, ke = kx - 1;
+  if (dir == 1) { is = 1; ie = ix - 1; }
+  if (dir == 2) { js = 1; je = jx - 1; }
+  if (dir == 3) { ks = 1; ke = kx - 1; }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = ks; k <= ke; k++)
+    for (int j = js; j <= je; j++)
+      for (int i = is; i <= ie; i++) {
+        const double eta_dx = eta_face[(size_t)(i - 1) + (size_t)ix * (size_t)(j - 1)] / dx;
+#define B(ii, jj, kk, m) U[IX4(ii, jj, kk, m)]
+        if (dir == 1) {
+          double JyS = -(B(i + 1, j, k, bz) - B(i, j, k, bz)) +
+                       0.25 * (B(i, j, k + 1, bx) + B(i + 1, j, k + 1, bx) - B(i, j, k - 1, bx) - B(i + 1, j, k - 1, bx));
+          double JzS = (B(i + 1, j, k, by) - B(i, j, k, by)) -
+                       0.25 * (B(i, j + 1, k, bx) + B(i + 1, j + 1, k, bx) - B(i, j - 1, k, bx) - B(i + 1, j - 1, k, bx));
+          F[IX4(i, j, k, en)] = F[IX4(i, j, k, en)] + 0.5 * eta_dx *
+                                (JyS * (B(i, j, k, bz) + B(i + 1, j, k, bz)) - JzS * (B(i, j, k, by) + B(i + 1, j, k, by)));
+          F[IX4(i, j, k, by)] = F[IX4(i, j, k, by)] - eta_dx * JzS;
+          F[IX4(i, j, k, bz)] = F[IX4(i, j, k, bz)] + eta_dx * JyS;
+        } else if (dir == 2) {
+          double JxS = (B(i, j + 1, k, bz) - B(i, j, k, bz)) -
+                       0.25 * (B(i, j, k + 1, by) + B(i, j + 1, k + 1, by) - B(i, j, k - 1, by) - B(i, j + 1, k - 1, by));
+          double JzS = -(B(i, j + 1, k, bx) - B(i, j, k, bx)) +
+                       0.25 * (B(i + 1, j, k, by) + B(i + 1, j + 1, k, by) - B(i - 1, j, k, by) - B(i - 1, j + 1, k, by));
+          F[IX4(i, j, k, en)] = F[IX4(i, j, k, en)] + 0.5 * eta_dx *
+                                (JzS * (B(i, j, k, bx) + B(i, j + 1, k, bx)) - JxS * (B(i, j, k, bz) + B(i, j + 1, k, bz)));
+          F[IX4(i, j, k, bx)] = F[IX4(i, j, k, bx)] + eta_dx * JzS;
+          F[IX4(i, j, k, bz)] = F[IX4(i, j, k, bz)] - eta_dx * JxS;
+        } else {
+          double JyS = (B(i, j, k + 1, bx) - B(i, j, k, bx)) -
+                       0.25 * (B(i + 1, j, k, bz) + B(i + 1, j, k + 1, bz) - B(i - 1, j, k, bz) - B(i - 1, j, k + 1, bz));
+          double JxS = -(B(i, j, k + 1, by) - B(i, j, k, by)) +
+                       0.25 * (B(i, j + 1, k, bz) + B(i, j + 1, k + 1, bz) - B(i, j - 1, k, bz) - B(i, j - 1, k + 1, bz));
+          F[IX4(i, j, k, en)] = F[IX4(i, j, k, en)] + 0.5 * eta_dx *
+                                (JxS * (B(i, j, k, by) + B(i, j, k + 1, by)) - JyS * (B(i, j, k, bx) + B(i, j, k + 1, bx)));
+          F[IX4(i, j, k, bx)] = F[IX4(i, j, k, bx)] - eta_dx * JyS;
+          F[IX4(i, j, k, by)] = F[IX4(i, j, k, by)] + eta_dx * JxS;
+        }
+#undef B
+      }
+}
+
+/* Localized resistivity of the 3D reconnection configuration at the face
+ * centres: eta = eta0 + eta1 * exp(-(x^2 + y^2) / w^2), x/y of the face
+ * centre as 3D_reconnection_gpu/flux_resistive.cuf:48-70 places them
+ * (x(i) + 0.5*dx on x-faces, y(j) + 0.5*dx on y-faces, cell centre on
+ * z-faces).  eta_out gets 3 arrays of ix*jx (x-, y-, z-faces); returns the
+ * maximum over faces, for set_dt_resistive. */
+double or_eta_faces(const double *x, const double *y, double dx, int ix, int jx, double eta0, double eta1, double w,
+                    double *eta_out)
+{
+  double emax = 0.0;
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < jx; j++)
+      for (int i = 0; i < ix; i++) {
+        double xx = x[i] + (d == 0 ? 0.5 * dx : 0.0);
+        double yy = y[j] + (d == 1 ? 0.5 * dx : 0.0);
+        double r2 = xx * xx + yy * yy;
+        double e = eta0 + eta1 * exp(-r2 / (w * w));
+        eta_out[(size_t)d * ix * jx + (size_t)i + (size_t)ix * j] = e;
+        if (e > emax) emax = e;
+      }
+  return emax;
+}
+
+/* set_dt_resistive, 3D_reconnection_gpu/flux_resistive.cuf:2-9, with the
+ * configured eta: dt = min(dt, 0.5*cfl*dx^2/eta_max). */
+double or_set_dt_resistive(double dt, double dx, double cfl, double eta_max)
+{
+  double dr = 0.5 * cfl * (dx * dx) / eta_max;
+  return dt < dr ? dt : dr;
+}
+
+/* ------------------------------------------------------------------------
  * One full step in the order of 2D_basic/main.f90:61-187 (2D) and
  * 3D_basic/main.f90 (3D):  u2v -> set_dt -> glm_ss2 -> per dimension
  * (9 x limiter -> face patch -> flux_solver) -> rk stage 1 -> bc(U1) ->
@@ -584,7 +667,9 @@ static int sweep(const or_problem *p, const double *Ust, const double *V, double
     if ((rc = or_limiter(Ust + m * n, VL + m * n, VR + m * n, ix, jx, kx, dir, p->lm_type))) return rc;
   int d = dir - 1;
   or_bc_face(VL, VR, ix, jx, kx, dir, p->bctype[2 * d], p->face_sgn + (2 * d) * 9, p->face_sgn + (2 * d + 1) * 9);
-  return or_flux_solver(FX, VL, VR, ch, ix, jx, kx, dir, p->flux_type, NULL);
+  if ((rc = or_flux_solver(FX, VL, VR, ch, ix, jx, kx, dir, p->flux_type, NULL))) return rc;
+  if (p->eta) or_flux_resistive(FX, Ust, p->eta + (size_t)d * ix * jx, p->dx, ix, jx, kx, dir); /* main.cuf:177-285 */
+  return 0;
 }
 
 int or_step(const or_problem *p, double *U, double *dt_out, double *ch_out, long *bad)
@@ -605,7 +690,10 @@ int or_step(const or_problem *p, double *U, double *dt_out, double *ch_out, long
   if (p->dt_fixed > 0) {
     dt = p->dt_fixed;
     ch = p->ch_fixed;
-  } else if ((rc = or_set_dt(U, V, &ch, &dt, p->dx, p->cfl, ix, jx, kx, nd, bad))) goto out;
+  } else {
+    if ((rc = or_set_dt(U, V, &ch, &dt, p->dx, p->cfl, ix, jx, kx, nd, bad))) goto out;
+    if (p->eta) dt = or_set_dt_resistive(dt, p->dx, p->cfl, p->eta_max); /* main.cuf:147 */
+  }
   or_glm_ss2(U, ch, dt, ix, jx, kx);
   for (int stage = 0; stage < 2; stage++) {
     const double *Ust = stage ? U1 : U;

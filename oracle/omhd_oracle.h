@@ -41,6 +41,8 @@ typedef struct or_problem {
   double face_sgn[54]; /* 6 faces x 9 slots: sign for the VL/VR face patch */
   double dt_fixed;     /* > 0: replay mode, use dt_fixed/ch_fixed instead of set_dt */
   double ch_fixed;
+  const double *eta;   /* NULL, or 3 face arrays (ix*jx each) of the resistivity */
+  double eta_max;
 } or_problem;
 
 int or_u2v(const double *U, double *V, int ix, int jx, int kx, long *bad);
@@ -56,6 +58,10 @@ int or_set_dt(const double *U, const double *V, double *vmax, double *dt, double
 void or_bc_u(double *U, int ix, int jx, int kx, const int *bctype, const double *sgn);
 void or_bc_face(double *VL, double *VR, int ix, int jx, int kx, int dir, int bctype, const double *sgn_lo,
                 const double *sgn_hi);
+void or_flux_resistive(double *F, const double *U, const double *eta_face, double dx, int ix, int jx, int kx, int dir);
+double or_eta_faces(const double *x, const double *y, double dx, int ix, int jx, double eta0, double eta1, double w,
+                    double *eta_out);
+double or_set_dt_resistive(double dt, double dx, double cfl, double eta_max);
 int or_step(const or_problem *p, double *U, double *dt, double *ch, long *bad);
 int or_run(const or_problem *p, double *U, int nsteps, double *dt_trace, double *ch_trace, long *bad);
 

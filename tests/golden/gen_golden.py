@@ -218,6 +218,29 @@ def main():
         out[f"U_{lm}"], out[f"dt_{lm}"] = U, dts
     save("limiters_run", **out)
 
+    # ---- small 3D periodic run (3D_basic/main.f90 order) --------------------
+    from omhd_testlib import ref_run_3d  # noqa: E402
+
+    U = smooth_state(np.random.default_rng(33), 14, 12, 10)
+    ref3.bc(U)
+    U0 = U.copy(order="F")
+    dts, chs = ref_run_3d(ref3, U, 4, 0.1, 0.4, 2)
+    save("run3d", U0=U0, U=U, dt=dts, ch=chs)
+
+
+def only_3d():
+    ref3 = Ref(3)
+    from omhd_testlib import ref_run_3d  # noqa: E402
+
+    U = smooth_state(np.random.default_rng(33), 14, 12, 10)
+    ref3.bc(U)
+    U0 = U.copy(order="F")
+    dts, chs = ref_run_3d(ref3, U, 4, 0.1, 0.4, 2)
+    save("run3d", U0=U0, U=U, dt=dts, ch=chs)
+
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "3d":
+        only_3d()
+    else:
+        main()

@@ -46,6 +46,7 @@ class OrProblem(C.Structure):
         ("bctype", C.c_int * 6),
         ("u_sgn", C.c_double * 54), ("face_sgn", C.c_double * 54),
         ("dt_fixed", C.c_double), ("ch_fixed", C.c_double),
+        ("eta", DP), ("eta_max", C.c_double),
     ]
 
 
@@ -139,6 +140,17 @@ class Oracle:
         bct = (C.c_int * 6)(*prob.bctype)
         sg = np.array(prob.u_sgn[:], dtype=np.float64)
         self.L.or_bc_u(ptr(U), ix, jx, kx, bct, ptr(sg))
+
+    def eta_faces(self, x, y, dx, eta0, eta1, w):
+        """3 face arrays (x-, y-, z-faces) of eta(x,y), Fortran (ix,jx) each."""
+        ix, jx = len(x), len(y)
+        out = np.zeros((3, jx, ix))
+        self.L.or_eta_faces.restype = C.c_double
+        self.L.or_eta_faces.argtypes = [DP, DP, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        DP]
+        emax = self.L.or_eta_faces(ptr(np.ascontiguousarray(x, dtype=np.float64)),
+                                   ptr(np.ascontiguousarray(y, dtype=np.float64)), dx, ix, jx, eta0, eta1, w, ptr(out))
+        return out, emax
 
     def run(self, prob, U, nsteps):
         dts = np.zeros(nsteps)
@@ -265,6 +277,51 @@ def ref_run_2d(ref: Ref, U, nsteps, dx, cfl, lm_type, bc="periodic", time_type=0
         sweep(U1, 2, G)
         ref.rk(1 if time_type == 0 else 3, U, U1, F, G, None, dt, dx)
         bcU(U)
+        ref.glm_ss2(U, ch, dt)
+        dts[s], chs[s] = dt, ch
+    return dts, chs
+
+
+def ref_run_3d(ref: Ref, U, nsteps, dx, cfl, lm_type, time_type=0, flux_type=3):
+    """One whole periodic 3D run sequenced like 3D_basic/main.f90:57-229,
+    calling only the reference routines."""
+    ix, jx, kx = U.shape[:3]
+    V = farr(ix, jx, kx, 4)
+    U1 = U.copy(order="F")
+    VL, VR, F, G, H = (farr(ix, jx, kx, 9) for _ in range(5))
+    dts, chs = np.zeros(nsteps), np.zeros(nsteps)
+
+    def sweep(Ust, dir, out):
+        for m in range(4):
+            ref.limiter(V[..., m], VL[..., m], VR[..., m], dir, lm_type)
+        for m in range(4, 9):
+            ref.limiter(Ust[..., m], VL[..., m], VR[..., m], dir, lm_type)
+        if dir == 1:
+            VR[ix - 2] = VR[0]
+            VL[0] = VL[ix - 2]
+        elif dir == 2:
+            VL[:, 0] = VL[:, jx - 2]
+            VR[:, jx - 2] = VR[:, 0]
+        else:
+            VL[:, :, 0] = VL[:, :, kx - 2]
+            VR[:, :, kx - 2] = VR[:, :, 0]
+        ref.flux_solver(out, VL, VR, ch, dir, flux_type)
+
+    for s in range(nsteps):
+        ref.u2v(U, V)
+        ch, dt = ref.set_dt(U, V, dx, cfl)
+        ref.glm_ss2(U, ch, dt)
+        sweep(U, 1, F)
+        sweep(U, 2, G)
+        sweep(U, 3, H)
+        ref.rk(0 if time_type == 0 else 2, U, U1, F, G, H, dt, dx)
+        ref.bc(U1)
+        ref.u2v(U1, V)
+        sweep(U1, 1, F)
+        sweep(U1, 2, G)
+        sweep(U1, 3, H)
+        ref.rk(1 if time_type == 0 else 3, U, U1, F, G, H, dt, dx)
+        ref.bc(U)
         ref.glm_ss2(U, ch, dt)
         dts[s], chs[s] = dt, ch
     return dts, chs

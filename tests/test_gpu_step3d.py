@@ -1,0 +1,74 @@
+"""The 3D resident step (XY pass + Z pass per stage) and the resistive flux
+against the oracle: bitwise dt sequence and state."""
+import numpy as np
+import pytest
+
+from omhd_testlib import make_problem, smooth_state
+from test_oracle_golden import bits_equal, load
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(oracle, U, nsteps, dx, lm=2, tt=0, bc=("periodic",) * 3, rows=0, resistivity=None, origin=(0.0, 0.0)):
+    from openmhd_amd import Solver, snapshot
+
+    ix, jx, kx = U.shape[:3]
+    p = make_problem(ix, jx, kx, dx=dx, cfl=0.4, lm_type=lm, time_type=tt, bc=bc)
+    eta = None
+    if resistivity is not None:
+        x = snapshot.grid(ix - 2, dx, origin[0])
+        y = snapshot.grid(jx - 2, dx, origin[1])
+        eta, emax = oracle.eta_faces(x, y, dx, *resistivity)
+        p.eta = eta.ctypes.data_as(p._fields_[-2][1])
+        p.eta_max = emax
+    oracle.bc_u(U, p)
+    s = Solver(ix - 2, jx - 2, dx, nz=kx - 2, cfl=0.4, lm_type=lm, time_type=tt, bc=bc, rows_per_wave=rows,
+               resistivity=resistivity, origin=origin)
+    s.upload(U)
+    dts, chs = s.step(nsteps, trace=True)
+    Ug = s.download()
+    s.close()
+    Uo = U.copy(order="F")
+    rc, dto, cho = oracle.run(p, Uo, nsteps)
+    assert rc == 0
+    return dts, dto, Ug, Uo
+
+
+@pytest.mark.parametrize("lm", (1, 2))
+@pytest.mark.parametrize("tt", (0, 1))
+def test_periodic(oracle, lm, tt):
+    U = smooth_state(np.random.default_rng(lm + 3 * tt), 70, 26, 22)
+    dts, dto, Ug, Uo = run_both(oracle, U, 4, 0.1, lm, tt, rows=7)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
+
+
+@pytest.mark.parametrize("walls", (("periodic", "wall", "periodic"), ("wall", "periodic", "wall")))
+def test_walls(oracle, walls):
+    U = smooth_state(np.random.default_rng(9), 66, 30, 20)
+    dts, dto, Ug, Uo = run_both(oracle, U, 4, 0.1, 1, 0, bc=walls, rows=9)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
+
+
+def test_golden_3d_run():
+    from openmhd_amd import Solver
+
+    d = load("run3d")
+    U = np.asfortranarray(d["U0"]).copy(order="F")
+    ix, jx, kx = U.shape[:3]
+    s = Solver(ix - 2, jx - 2, 0.1, nz=kx - 2, cfl=0.4, lm_type=2)
+    s.upload(U)
+    dts, chs = s.step(4, trace=True)
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"]) and bits_equal(s.download(), d["U"])
+
+
+@pytest.mark.parametrize("bc", (("periodic", "wall", "periodic"), ("wall", "wall", "periodic")))
+def test_resistive_harris(oracle, bc):
+    """Config C5's physics (Harris sheet, localized eta, HLLD + resistive flux +
+    GLM + TVD-RK2, resistive dt limit) on a small grid."""
+    from openmhd_amd import problems
+
+    dx = 0.25
+    U, origin = problems.harris_sheet_3d(68, 40, 24, dx, noise=1e-2)
+    dts, dto, Ug, Uo = run_both(oracle, U, 5, dx, 1, 0, bc=bc, rows=11, resistivity=(1e-3, 1e-2, 2.0),
+                                origin=origin)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)

@@ -130,3 +130,44 @@ def test_orszag_tang_1000_steps(oracle):
     assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
     interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
     assert hashlib.sha256(interior.tobytes()).hexdigest() == str(d["final_sha256"])
+
+
+def test_3d_periodic_run(oracle):
+    """3D periodic (3D_basic/main.f90 sequence), MC, 4 steps, bitwise to the
+    reference routines."""
+    d = load("run3d")
+    U = np.asfortranarray(d["U0"]).copy(order="F")
+    ix, jx, kx = U.shape[:3]
+    rc, dts, chs = oracle.run(make_problem(ix, jx, kx, dx=0.1, cfl=0.4, lm_type=2), U, 4)
+    assert rc == 0
+    assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"]) and bits_equal(U, d["U"])
+
+
+def test_resistive_closed_form(oracle):
+    """Oracle flux_resistive on B = (0, 0, a x): J_y = -a, so the x-face flux of
+    Bz is eta*J_y = -eta*a and the energy flux is (E x B)_x = eta*J_y*Bz; uniform
+    B gives no resistive flux (3D_reconnection_gpu/flux_resistive.cuf:89-101)."""
+    import ctypes as C
+
+    from omhd_testlib import ptr
+
+    ix, jx, kx, dx, a = 8, 6, 5, 0.5, 0.3
+    U = np.zeros((ix, jx, kx, 9), order="F")
+    U[..., 4] = 1.0
+    x = (np.arange(ix) - 0.5) * dx
+    U[..., 7] = a * x[:, None, None]
+    eta = np.full((jx, ix), 0.02)
+    F = np.zeros((ix, jx, kx, 9), order="F")
+    oracle.L.or_flux_resistive.argtypes = [C.POINTER(C.c_double)] * 3 + [C.c_double] + [C.c_int] * 4
+    oracle.L.or_flux_resistive(ptr(F), ptr(U), ptr(eta), dx, ix, jx, kx, 1)
+    inner = F[0:ix - 1, 1:jx - 1, 1:kx - 1]
+    np.testing.assert_allclose(inner[..., 7], -0.02 * a, rtol=1e-14)
+    bz_face = 0.5 * (U[0:ix - 1, 1:jx - 1, 1:kx - 1, 7] + U[1:ix, 1:jx - 1, 1:kx - 1, 7])
+    np.testing.assert_allclose(inner[..., 3], -0.02 * a * bz_face, rtol=1e-13, atol=1e-16)
+    assert np.all(inner[..., [0, 1, 2, 4, 5, 6, 8]] == 0)
+    U[..., 7] = 0.7  # uniform field: no current
+    U[..., 5] = -0.2
+    F[...] = 0
+    for d in (1, 2, 3):
+        oracle.L.or_flux_resistive(ptr(F), ptr(U), ptr(eta), dx, ix, jx, kx, d)
+    assert np.all(F == 0)
