@@ -84,6 +84,87 @@ def cpu_baseline(n: int, sample_steps: int):
                       f"(gcc -O2 -fopenmp, {threads} threads)"}
 
 
+def bench_ensemble(a):
+    """Config C3: 256 independent KH runs, sides log-uniform in [64, 1024],
+    300 steps, each with its own dt.  Batched (one launch per RK stage over all
+    members, autotuned tiles; host dt and device dt + hipGraph) against the same
+    members run one after another with the single-grid step."""
+    import torch
+
+    from openmhd_amd import Ensemble, Solver, make_config, problems
+
+    torch.cuda.set_device(0)
+    params = problems.kh_ensemble(a.members)
+    states, cfgs = [], []
+    for p in params:
+        U, dx = problems.kh_member(p)
+        states.append(U)
+        cfgs.append(make_config(p["nx"], p["ny"], dx, cfl=0.4, lm_type=1, bc=("periodic", "wall")))
+    cells = sum(p["nx"] * p["ny"] for p in params)
+    log(f"ensemble: {len(params)} members, {cells} cells")
+    steps = a.steps
+    seq = 0.0
+    for U, c in zip(states, cfgs):
+        s = Solver(c.nx, c.ny, c.dx, cfl=0.4, lm_type=1, bc=("periodic", "wall"))
+        s.upload(U)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        seq += time.perf_counter() - t0
+        s.close()
+    log(f"one after another: {seq:.3f} s")
+    out = {"members": len(params), "cells": cells, "steps": steps, "sequential_s": seq}
+    for mode in ("host_dt", "device_dt_graph"):
+        e = Ensemble(cfgs)
+        for i, U in enumerate(states):
+            e.upload(i, U)
+        if mode != "host_dt":
+            e.set_mode(device_dt=True, graph=True)
+        rows, grid = e.autotune(3)
+        e.step(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.step(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        e.close()
+        log(f"batched {mode}: {el:.3f} s (rows {rows}, grid {grid})")
+        out[mode] = {"seconds": el, "zone_updates_per_s": cells * steps / el, "speedup_vs_sequential": seq / el,
+                     "tile_rows": rows, "grid": grid}
+    out["sequential_zone_updates_per_s"] = cells * steps / seq
+    print(json.dumps({"workload": "C3 KH ensemble", **out}), flush=True)
+
+
+def bench_3d(a):
+    """Config C5: 3D resistive Harris sheet 512x256x256, HLLD + resistive flux
+    + GLM + TVD-RK2, minmod, x/z periodic, y walls."""
+    import torch
+
+    from openmhd_amd import Solver, problems
+
+    torch.cuda.set_device(0)
+    nx, ny, nz, dx = 512, 256, 256, 0.125
+    U, origin = problems.harris_sheet_3d(nx, ny, nz, dx)
+    s = Solver(nx, ny, dx, nz=nz, cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"),
+               resistivity=(1e-3, 1e-2, 2.0), origin=origin)
+    s.upload(U)
+    del U
+    s.step(a.warmup)
+    s.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms, n = s.timing()
+    s.close()
+    cells = nx * ny * nz
+    print(json.dumps({"workload": "C5 3D resistive Harris 512x256x256", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
+                      "zone_updates_per_s": cells * a.steps / el,
+                      "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+                      "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,12 +174,19 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d"),
+                    help="kh: the headline line; ensemble / 3d: configurations C3 / C5 (own JSON lines)")
+    ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py: this build runs on one GPU only (the grid is one domain on one MI355X); "
                  "multi-GPU sharding is a design note, not built -- run with --gpus 1")
+
+    if a.workload != "kh":
+        sys.path.insert(0, str(REPO))
+        return bench_ensemble(a) if a.workload == "ensemble" else bench_3d(a)
 
     # the CPU baseline runs first, as a child process started before this
     # process initialises the GPU
