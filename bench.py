@@ -165,6 +165,102 @@ def bench_3d(a):
                       "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}), flush=True)
 
 
+def bench_adjoint(a):
+    """Config C4: Orszag-Tang 512^2 + 16 seeded Fourier velocity modes
+    (|k| <= 8, amplitude 1e-2), 200 MC/HLLD/GLM/TVD-RK2 steps, checkpoints
+    every 20; loss = ||By(T) - By_target(T)||^2 with the target run from a
+    second seeded perturbation; gradient w.r.t. the 16 mode amplitudes via the
+    adjoint (torch.autograd through Rollout), checked by central differences
+    along 16 seeded directions with the frozen dt/ch sequence."""
+    import numpy as np
+    import torch
+
+    from openmhd_amd import Adjoint, Rollout, Solver, problems
+
+    torch.cuda.set_device(0)
+    n, nsteps, every = a.n if a.n != 4096 else 512, a.steps if a.steps != 500 else 200, 20
+    dx = 2 * np.pi / n
+    s = Solver(n, n, dx, cfl=0.4, lm_type=2)
+    modes = problems.fourier_modes(1)
+    tmodes = problems.fourier_modes(2)
+    tgt_coef = np.full(16, 1e-2) * np.random.default_rng(2).uniform(0.5, 1.5, 16)
+    s.set_state(problems.orszag_tang_perturbed(n, tmodes, tgt_coef))
+    s.step(nsteps)
+    target = torch.tensor(s.get_state()[6], device="cuda")
+
+    def loss_of(coef, roll, dt=None, ch=None):
+        W0 = problems.orszag_tang_perturbed(n, modes, coef)
+        WT = roll(W0, dt, ch)
+        return ((WT[6] - target) ** 2).sum()
+
+    roll = Rollout(s, nsteps, every)
+    coef = torch.full((16,), 1e-2, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = loss_of(coef, roll)
+    L.backward()
+    grad = coef.grad.detach().cpu().numpy()
+    dts, chs = roll.dt.copy(), roll.ch.copy()
+    # central differences of the forward with the frozen sequence
+    # (the map is piecewise smooth: a larger step crosses limiter kinks, a
+    # smaller one sees the forward's rounding noise; the sweep shows both)
+    rng = np.random.default_rng(16)
+    eps_list = (1e-7, 1e-8, 1e-9, 1e-10)
+    rel = []
+    with torch.no_grad():
+        for k in range(16):
+            d = rng.standard_normal(16)
+            d /= np.linalg.norm(d)
+            dv = torch.tensor(d, device="cuda")
+            an = float(grad @ d)
+            r = []
+            for eps in eps_list:
+                lp = float(loss_of(coef + eps * dv, roll, dts, chs))
+                lm = float(loss_of(coef - eps * dv, roll, dts, chs))
+                r.append(abs((lp - lm) / (2 * eps) - an) / abs(an))
+            rel.append(r)
+            log(f"direction {k}: adjoint {an:.12e} rel. diff. at eps {eps_list}: " + " ".join(f"{x:.1e}" for x in r))
+    rel = np.array(rel)
+    best = rel.min(axis=1)
+    # timing: plain steps, the adjoint's forward pass (with checkpoints), the backward pass
+    W0 = problems.orszag_tang_perturbed(n, modes, np.full(16, 1e-2))
+    W0d = torch.tensor(W0, device="cuda")
+    lamT = torch.randn_like(W0d)
+    adj = Adjoint(s, nsteps, every)
+    t = {}
+    for rep in range(2):  # the first round warms up
+        s.set_state(W0d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step_fixed(dts, chs)
+        torch.cuda.synchronize()
+        t["plain_fixed"] = time.perf_counter() - t0
+        s.set_state(W0d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(nsteps)
+        torch.cuda.synchronize()
+        t["plain_cfl"] = time.perf_counter() - t0
+        s.set_state(W0d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        adj.forward(dts, chs)
+        torch.cuda.synchronize()
+        t["forward"] = time.perf_counter() - t0
+        out = torch.empty_like(W0d)
+        t0 = time.perf_counter()
+        adj.backward(lamT, out)
+        torch.cuda.synchronize()
+        t["backward"] = time.perf_counter() - t0
+    adj.close()
+    res = {"workload": f"C4 adjoint OT {n}^2 + 16 modes, {nsteps} steps, checkpoints every {every}",
+           "loss": float(L), "grad_norm": float(np.linalg.norm(grad)), "fd_eps": list(eps_list),
+           "fd_rel_err_median_per_eps": [float(x) for x in np.median(rel, axis=0)],
+           "fd_rel_err_best_max": float(best.max()), "fd_rel_err_best_median": float(np.median(best)),
+           "seconds": t, "backward_over_forward": t["backward"] / t["forward"],
+           "backward_over_plain_step": t["backward"] / t["plain_cfl"],
+           "zone_updates_per_s_forward": n * n * nsteps / t["forward"]}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,8 +270,8 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d"),
-                    help="kh: the headline line; ensemble / 3d: configurations C3 / C5 (own JSON lines)")
+    ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d", "adjoint"),
+                    help="kh: the headline line; ensemble / adjoint / 3d: configurations C3 / C4 / C5 (own JSON lines)")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
@@ -186,7 +282,7 @@ def main():
 
     if a.workload != "kh":
         sys.path.insert(0, str(REPO))
-        return bench_ensemble(a) if a.workload == "ensemble" else bench_3d(a)
+        return {"ensemble": bench_ensemble, "3d": bench_3d, "adjoint": bench_adjoint}[a.workload](a)
 
     # the CPU baseline runs first, as a child process started before this
     # process initialises the GPU

@@ -137,6 +137,12 @@ int omhd_upload(omhd_handle *h, const double *U);
  * number of completed steps in *done (optional). */
 int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done);
 
+/* Advance nsteps steps with the given dt[n] and ch[n] instead of the CFL
+ * values (the reference's loop with set_dt replaced by a fixed sequence): the
+ * frozen-dt forward map the adjoint differentiates, and the replay of a
+ * recorded run.  No counterpart routine in the reference. */
+int omhd_step_fixed(omhd_handle *h, int32_t nsteps, const double *dt, const double *ch);
+
 /* Copy the state back as U(ix,jx[,kx],9) with ghosts filled by the boundary
  * conditions, and optionally V(ix,jx[,kx],4) (primitives of every cell). */
 int omhd_download(omhd_handle *h, double *U, double *V);
@@ -144,6 +150,12 @@ int omhd_download(omhd_handle *h, double *U, double *V);
 /* Device pointer of the state (SoA, 9 planes) and its layout, for callers that
  * keep the data on the GPU (the PyTorch binding, the benchmark). */
 int omhd_state_layout(omhd_handle *h, double **dev_state, int64_t *plane_stride, int32_t *row_pitch);
+
+/* The state as W(9, nz, ny, nx) -- interior cells only, x fastest, slots in
+ * the order of param.h:6-15 -- from or to host or device memory (the layout
+ * of the PyTorch binding).  omhd_set_state also computes the CFL speed. */
+int omhd_set_state(omhd_handle *h, const double *W);
+int omhd_get_state(omhd_handle *h, double *W);
 
 /* HIP stream the handle's kernels run on (as void*: a hipStream_t). */
 void *omhd_stream(omhd_handle *h);
@@ -179,6 +191,22 @@ int omhd_batch_time(omhd_batch *b, double *t);
  * ensemble state is restored after every trial. */
 int omhd_batch_autotune(omhd_batch *b, int32_t trial_steps, int32_t *rows, int32_t *grid);
 int omhd_batch_destroy(omhd_batch *b);
+
+/* ---- discrete adjoint of the 2D step ------------------------------------- */
+/* No counterpart in the reference.  The reverse-mode derivative of nsteps
+ * steps of h (frozen dt/ch sequence) with respect to the initial state:
+ * hand-written reverse kernels of u2v, the limiter (the branch the forward
+ * pass took), HLLD, glm_ss2 and the RK stages.  omhd_adjoint_forward runs the
+ * steps from h's current state (with the CFL dt, recorded into dt_out/ch_out,
+ * or with dt_in/ch_in imposed), keeping the state every `every` steps;
+ * omhd_adjoint_backward maps lam_T = dL/dU(T) to lam_0 = dL/dU(0), replaying
+ * each segment from its checkpoint.  lam_T and lam_0 use the W(9, nz, ny, nx)
+ * layout of omhd_set_state, host or device memory.  Periodic 2D grids. */
+typedef struct omhd_adjoint omhd_adjoint;
+int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every, omhd_adjoint **a);
+int omhd_adjoint_forward(omhd_adjoint *a, const double *dt_in, const double *ch_in, double *dt_out, double *ch_out);
+int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, double *lam_0);
+int omhd_adjoint_destroy(omhd_adjoint *a);
 
 /* Library version and the gfx target it was built for. */
 const char *omhd_version(void);

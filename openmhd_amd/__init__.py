@@ -7,9 +7,12 @@ thin ctypes layer over its C ABI (include/omhd.h):
                        glm_ss2, rk_tvd21/22, rk_std21/22, set_dt, bc, bc_for_*)
   openmhd_amd.solver   Solver (2D/3D): resident state, fused whole-step kernels
   openmhd_amd.problems initial states of the benchmark configurations
+  openmhd_amd.adjoint  discrete adjoint of the 2D step, torch.autograd.Function
 """
 from ._lib import LIB_PATH, OMHDError, lib  # noqa: F401  (raises ImportError if libomhd is not built)
-from . import api, problems, solver  # noqa: F401
+from . import adjoint, api, problems, solver  # noqa: F401
+from .adjoint import Adjoint, Rollout  # noqa: F401
 from .solver import Ensemble, Solver, Solver2D, make_config  # noqa: F401
 
-__all__ = ["api", "solver", "problems", "Solver", "Solver2D", "Ensemble", "make_config", "OMHDError", "LIB_PATH"]
+__all__ = ["api", "solver", "problems", "adjoint", "Solver", "Solver2D", "Ensemble", "Adjoint", "Rollout", "make_config",
+           "OMHDError", "LIB_PATH"]

@@ -61,6 +61,14 @@ class OMHDError(RuntimeError):
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"libomhd is not built: {LIB_PATH} missing (run __graft_entry__.build())")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 with the
+    # same soname (libamdhip64.so.7) as ROCm's.  Whichever is loaded first
+    # serves both, and PyTorch's build does not start on ROCm's newer copy, so
+    # load PyTorch's (when installed) before libomhd binds to one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     sig = {
         "omhd_last_error": [C.POINTER(ErrorInfo)],
@@ -88,6 +96,13 @@ def _load() -> C.CDLL:
         "omhd_batch_autotune": [C.c_void_p, I32, C.POINTER(I32), C.POINTER(I32)],
         "omhd_batch_destroy": [C.c_void_p],
         "omhd_get_timing": [C.c_void_p, DP, C.POINTER(C.c_int64)],
+        "omhd_step_fixed": [C.c_void_p, I32, DP, DP],
+        "omhd_set_state": [C.c_void_p, DP],
+        "omhd_get_state": [C.c_void_p, DP],
+        "omhd_adjoint_create": [C.c_void_p, I32, I32, C.POINTER(C.c_void_p)],
+        "omhd_adjoint_forward": [C.c_void_p, DP, DP, DP, DP],
+        "omhd_adjoint_backward": [C.c_void_p, DP, DP],
+        "omhd_adjoint_destroy": [C.c_void_p],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -33,6 +33,7 @@
 
 #include "omhd_device.h"
 #include "omhd_host.h"
+#include "omhd_internal.h"
 
 namespace omhd {
 namespace {
@@ -459,10 +460,6 @@ __global__ void k_cfl(const double *__restrict__ s, long long plane, long long z
 }
 
 // Fortran array U(ix,jx[,kx],9) (+ V(..,4)) with ghosts <- SoA interior
-struct Faces {
-  int bc[3];
-  unsigned slo[3], shi[3];
-};
 
 __global__ void k_to_fortran(const double *__restrict__ s, long long plane, long long zstride, int pitch, int nx,
                              int ny, int nz, int nd, Faces f, double *__restrict__ U, double *__restrict__ V) {
@@ -547,30 +544,6 @@ unsigned sign_mask(const double *s9) {
 }  // namespace omhd
 
 using namespace omhd;
-
-struct omhd_handle {
-  omhd_config cfg;
-  int nd = 2;
-  int pitch = 0;
-  long long zstride = 0, plane = 0;
-  double *U = nullptr;    // U^n / U^{n+1}
-  double *U1 = nullptr;   // U*
-  double *acc = nullptr;  // 3D: partial flux difference
-  double *eta = nullptr;  // resistive face arrays
-  double eta_max = 0.0;
-  unsigned long long *red = nullptr;       // [0] vmax bits, [1..2] error (4 ints)
-  unsigned long long *red_host = nullptr;  // pinned mirror
-  hipStream_t stream = nullptr;
-  double vmax = -1.0;  // CFL speed of the current state
-  bool have_state = false;
-  Faces faces{};
-  int rows = 24;
-  // optional per-stage timing with HIP events on the handle's stream
-  bool timing = false;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  double stage_ms[2] = {0.0, 0.0};
-  long long stage_launches[2] = {0, 0};
-};
 
 static int read_red(omhd_handle *h, double &vmax, int err[4]) {
   OMHD_HIP(hipMemcpyAsync(h->red_host, h->red, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
@@ -779,65 +752,69 @@ static Sweep base_sweep(omhd_handle *h, int pass) {
 
 static unsigned grid_of(const Sweep &p) { return (unsigned)((p.ntiles + 7) / 8 * 8); }
 
-extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
-  if (done) *done = 0;
-  if (!h) return set_error(OMHD_ERR_ARG, "null handle");
-  if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
+namespace omhd {
+
+int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, double *out, double dt, double ch) {
   const omhd_config &c = h->cfg;
   const bool three = h->nd == 3;
   const int res = c.resistive ? 1 : 0;
   Sweep pa = base_sweep(h, three ? PASS_XY : PASS_2D);
   Sweep pz = three ? base_sweep(h, PASS_Z) : pa;
-  const march_fn a1 = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, 1, c.time_type);
-  const march_fn a2 = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, 2, c.time_type);
-  const march_fn z1 = three ? march_kernel(PASS_Z, res, c.lm_type, 1, c.time_type) : nullptr;
-  const march_fn z2 = three ? march_kernel(PASS_Z, res, c.lm_type, 2, c.time_type) : nullptr;
-  const double cr = 0.2;  // glm_ss2.f90:18
+  const march_fn a = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, stage, c.time_type);
+  const march_fn z = three ? march_kernel(PASS_Z, res, c.lm_type, stage, c.time_type) : nullptr;
+  const double cr = 0.2;                          // glm_ss2.f90:18
+  const double glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
+  // stage 1: U* = U + dtx dF (rk_tvd21: dt/dx, rk_std21: 0.5*dt/dx); stage 2: dt/dx
+  const double dtx = (stage == 1 && c.time_type != 0) ? 0.5 * dt / c.dx : dt / c.dx;
+  for (Sweep *q : {&pa, &pz}) {
+    q->ch = ch;
+    q->ch2 = ch * ch;
+    q->glm_f = glm_f;
+    q->us = us;
+    q->u0 = u0;
+    q->out = out;
+    q->dtx = dtx;
+  }
+  hipLaunchKernelGGL(a, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
+  OMHD_HIP(hipGetLastError());
+  if (three) {
+    hipLaunchKernelGGL(z, dim3(grid_of(pz)), dim3(64), 0, h->stream, pz);
+    OMHD_HIP(hipGetLastError());
+  }
+  return OMHD_OK;
+}
+
+int refresh_cfl(omhd_handle *h) { return cfl_pass(h); }
+
+}  // namespace omhd
+
+// the step's dt from the current CFL speed (set_dt.f90:52, set_dt_resistive)
+static int step_dt(omhd_handle *h, double &dt, double &ch) {
+  const omhd_config &c = h->cfg;
+  ch = h->vmax;           // set_dt returns vmax as ch (main.f90:80)
+  dt = c.cfl * c.dx / h->vmax;
+  if (!(dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, dt);  // set_dt.f90:55-65
+  if (c.resistive) {  // set_dt_resistive (flux_resistive.cuf:2-9) with the configured eta
+    const double dr = 0.5 * c.cfl * (c.dx * c.dx) / h->eta_max;
+    dt = dt < dr ? dt : dr;
+  }
+  return OMHD_OK;
+}
+
+extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
+  if (done) *done = 0;
+  if (!h) return set_error(OMHD_ERR_ARG, "null handle");
+  if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
   for (int n = 0; n < nsteps; n++) {
-    const double ch = h->vmax;           // set_dt returns vmax as ch (main.f90:80)
-    double dt = c.cfl * c.dx / h->vmax;  // set_dt.f90:52
-    if (!(dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, dt);  // set_dt.f90:55-65
-    if (res) {  // set_dt_resistive (flux_resistive.cuf:2-9) with the configured eta
-      const double dr = 0.5 * c.cfl * (c.dx * c.dx) / h->eta_max;
-      dt = dt < dr ? dt : dr;
-    }
-    const double glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
-    for (Sweep *q : {&pa, &pz}) {
-      q->ch = ch;
-      q->ch2 = ch * ch;
-      q->glm_f = glm_f;
-    }
+    double dt, ch;
+    if (int rc = step_dt(h, dt, ch)) return rc;
     k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[0], h->stream));
-    // stage 1: U* = U + dtx dF   (rk_tvd21: dt/dx, rk_std21: 0.5*dt/dx)
-    const double dtx1 = (c.time_type == 0) ? dt / c.dx : 0.5 * dt / c.dx;
-    for (Sweep *q : {&pa, &pz}) {
-      q->us = h->U;
-      q->u0 = h->U;
-      q->out = h->U1;
-      q->dtx = dtx1;
-    }
-    hipLaunchKernelGGL(a1, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
-    OMHD_HIP(hipGetLastError());
-    if (three) {
-      hipLaunchKernelGGL(z1, dim3(grid_of(pz)), dim3(64), 0, h->stream, pz);
-      OMHD_HIP(hipGetLastError());
-    }
+    if (int rc = stage_launch(h, 1, h->U, h->U, h->U1, dt, ch)) return rc;
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[1], h->stream));
     // stage 2: U^{n+1} in place over U^n (each cell's U^n is read only by the
     // wavefront that overwrites it, just before it does)
-    for (Sweep *q : {&pa, &pz}) {
-      q->us = h->U1;
-      q->u0 = h->U;
-      q->out = h->U;
-      q->dtx = dt / c.dx;
-    }
-    hipLaunchKernelGGL(a2, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
-    OMHD_HIP(hipGetLastError());
-    if (three) {
-      hipLaunchKernelGGL(z2, dim3(grid_of(pz)), dim3(64), 0, h->stream, pz);
-      OMHD_HIP(hipGetLastError());
-    }
+    if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, dt, ch)) return rc;
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[2], h->stream));
     int err[4];
     double vm;
@@ -858,6 +835,68 @@ extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_t
     if (ch_trace) ch_trace[n] = ch;
     if (done) *done = n + 1;
   }
+  return ok();
+}
+
+extern "C" int omhd_step_fixed(omhd_handle *h, int32_t nsteps, const double *dt, const double *ch) {
+  if (!h || (nsteps > 0 && (!dt || !ch))) return set_error(OMHD_ERR_ARG, "null argument");
+  if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
+  for (int n = 0; n < nsteps; n++)
+    if (!(dt[n] > 0) || !(ch[n] >= 0)) return set_error(OMHD_ERR_ARG, "dt must be > 0 and ch >= 0", n);
+  k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
+  for (int n = 0; n < nsteps; n++) {
+    if (int rc = stage_launch(h, 1, h->U, h->U, h->U1, dt[n], ch[n])) return rc;
+    if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, dt[n], ch[n])) return rc;
+  }
+  int err[4];
+  double vm;
+  if (int rc = read_red(h, vm, err)) return rc;
+  if (err[0]) return state_error(err);
+  if (int rc = cfl_pass(h)) return rc;  // the speed of the final state, for a later omhd_step
+  return ok();
+}
+
+__global__ void k_state_copy(double *__restrict__ dst, const double *__restrict__ src, long long plane, long long zstride,
+                             int pitch, int nx, int ny, int nz, int to_compact) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)nx * ny * nz;
+  if (gid >= n) return;
+  const int i = (int)(gid % nx);
+  const long long r = gid / nx;
+  const int j = (int)(r % ny), k = (int)(r / ny);
+  const long long o = (long long)k * zstride + (long long)j * pitch + i;
+  for (int m = 0; m < NV; m++) {
+    if (to_compact) dst[m * n + gid] = src[m * plane + o];
+    else dst[m * plane + o] = src[m * n + gid];
+  }
+}
+
+extern "C" int omhd_set_state(omhd_handle *h, const double *W) {
+  if (!h || !W) return set_error(OMHD_ERR_ARG, "null argument");
+  const omhd_config &c = h->cfg;
+  Staged a;
+  if (int rc = a.init(W, 9 * ncells(h), true, false, h->stream)) return rc;
+  k_state_copy<<<(unsigned)((ncells(h) + 255) / 256), 256, 0, h->stream>>>(h->U, a.dev, h->plane, h->zstride,
+                                                                           h->pitch, c.nx, c.ny, c.nz, 0);
+  OMHD_HIP(hipGetLastError());
+  a.release(h->stream);
+  if (int rc = cfl_pass(h)) return rc;
+  h->have_state = true;
+  return ok();
+}
+
+extern "C" int omhd_get_state(omhd_handle *h, double *W) {
+  if (!h || !W) return set_error(OMHD_ERR_ARG, "null argument");
+  if (!h->have_state) return set_error(OMHD_ERR_ARG, "no state uploaded");
+  const omhd_config &c = h->cfg;
+  Staged a;
+  if (int rc = a.init(W, 9 * ncells(h), false, true, h->stream)) return rc;
+  k_state_copy<<<(unsigned)((ncells(h) + 255) / 256), 256, 0, h->stream>>>(a.dev, h->U, h->plane, h->zstride,
+                                                                           h->pitch, c.nx, c.ny, c.nz, 1);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = a.finish(h->stream)) return rc;
+  a.release(h->stream);
+  OMHD_HIP(hipStreamSynchronize(h->stream));
   return ok();
 }
 

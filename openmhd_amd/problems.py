@@ -165,3 +165,63 @@ def kh_member(p: dict):
     """Initial state of one ensemble member: the KH layer of kelvin_helmholtz
     on an nx x ny grid of the member's parameters (Lx = 40, y extent ny*dx)."""
     return kelvin_helmholtz(p["nx"], p["ny"], seed=p["seed"], u0=p["u0"], beta=p["beta"], bx=p["bx"])
+
+
+def fourier_modes(seed: int, n_modes: int = 16, kmax: int = 8):
+    """A seeded set of in-plane velocity modes for the adjoint configuration:
+    integer wave vectors k with 1 <= |k| <= kmax, phases phi in [0, 2pi) and
+    unit directions e in the (vx, vy) plane.  dv(x) = sum_m c_m e_m sin(k_m.x
+    + phi_m) on the [0, 2pi)^2 Orszag-Tang box."""
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < n_modes:
+        k = rng.integers(-kmax, kmax + 1, 2)
+        if 1 <= np.hypot(*k) <= kmax:
+            ks.append(k)
+    th = rng.uniform(0, 2 * np.pi, n_modes)
+    return {"k": np.array(ks, dtype=np.float64), "phi": rng.uniform(0, 2 * np.pi, n_modes),
+            "e": np.stack([np.cos(th), np.sin(th)], axis=1)}
+
+
+def orszag_tang_perturbed(n: int, modes, coef, fixed: str = "energy"):
+    """Interior Orszag-Tang state W(9, n, n) (x fastest, conserved variables)
+    with the velocity perturbation of `modes` scaled by `coef` (16 values);
+    numpy, or torch (then differentiable in coef).  Density and B are those of
+    the vortex.  fixed="energy" keeps the total energy of the vortex (the
+    kinetic energy change comes out of the pressure, so p is not uniform);
+    fixed="pressure" keeps p = gamma uniform.  The uniform-pressure state sits
+    on the limiters' kink for every cell (zero pressure slopes), where the
+    step has direction-dependent one-sided derivatives and no gradient; the
+    fixed-energy state does not."""
+    dx = 2 * np.pi / n
+    x = (np.arange(n) + 0.5) * dx  # cell centres of model.f90:19-24 (x(i) = -dx/2 + (i-1) dx)
+    X, Y = np.meshgrid(x, x, indexing="xy")  # X[j, i] = x_i
+    basis = np.stack([np.sin(k[0] * X + k[1] * Y + p) for k, p in zip(modes["k"], modes["phi"])])  # (M, n, n)
+    ex, ey = modes["e"][:, 0], modes["e"][:, 1]
+    fields = {"vx": -np.sin(Y), "vy": np.sin(X), "bx": -np.sin(Y), "by": np.sin(2 * X)}
+    is_torch = hasattr(coef, "requires_grad")
+    if is_torch:
+        import torch
+
+        def T(a):
+            return torch.as_tensor(a, device=coef.device, dtype=torch.float64)
+
+        basis, ex, ey = T(basis), T(ex), T(ey)
+        fields = {k: T(v) for k, v in fields.items()}
+        dvx = torch.tensordot(coef * ex, basis, dims=1)
+        dvy = torch.tensordot(coef * ey, basis, dims=1)
+        stack, zeros = torch.stack, torch.zeros_like
+    else:
+        dvx = np.tensordot(coef * ex, basis, axes=1)
+        dvy = np.tensordot(coef * ey, basis, axes=1)
+        stack, zeros = np.stack, np.zeros_like
+    ro, p = GAMMA ** 2, GAMMA
+    vx, vy = fields["vx"] + dvx, fields["vy"] + dvy
+    bx, by = fields["bx"], fields["by"]
+    z = zeros(vx)
+    if fixed == "energy":
+        v0x, v0y = fields["vx"], fields["vy"]
+        en = 0.5 * (ro * (v0x * v0x + v0y * v0y) + (bx * bx + by * by)) + p / (GAMMA - 1) + z
+    else:
+        en = 0.5 * (ro * (vx * vx + vy * vy) + (bx * bx + by * by)) + p / (GAMMA - 1)
+    return stack([ro * vx, ro * vy, z, en, z + ro, bx, by, z, z])

@@ -13,6 +13,23 @@ from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, Config, check, lib
 from .api import KH_WALL_SIGN, _ptr
 
 
+def _cptr(a, n_expected):
+    """Pointer of a C-ordered float64 array (numpy, or torch on host/device)."""
+    if hasattr(a, "data_ptr"):
+        import torch
+
+        if a.dtype != torch.float64 or not a.is_contiguous():
+            raise TypeError("contiguous float64 tensor required")
+        if a.numel() != n_expected:
+            raise ValueError(f"tensor has {a.numel()} elements, expected {n_expected}")
+        return C.cast(C.c_void_p(a.data_ptr()), DP)
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("C-contiguous float64 numpy array required")
+    if a.size != n_expected:
+        raise ValueError(f"array has {a.size} elements, expected {n_expected}")
+    return a.ctypes.data_as(DP)
+
+
 def make_config(nx, ny, dx, nz=0, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic", "periodic"),
                 wall_sign=KH_WALL_SIGN, rows_per_wave=0, resistivity=None, origin=(0.0, 0.0)):
     """omhd_config for a grid (see Solver)."""
@@ -88,6 +105,32 @@ class Solver:
         self.t = t.value
         check(rc, f"omhd_step (after {done.value} steps)")
         return (dts, chs) if trace else None
+
+    def state_shape(self):
+        """Shape of the interior state W: (9, [nz,] ny, nx), x fastest."""
+        return (9, self.nz, self.ny, self.nx) if self.ndim == 3 else (9, self.ny, self.nx)
+
+    def set_state(self, W):
+        """Load W (state_shape(), C order: interior cells, slots of param.h)
+        from host (numpy) or device (torch) memory; computes the CFL speed."""
+        check(lib.omhd_set_state(self.h, _cptr(W, int(np.prod(self.state_shape())))), "omhd_set_state")
+
+    def get_state(self, out=None):
+        """The interior state as W (state_shape()); into `out` (numpy or torch,
+        host or device) when given, else a new numpy array."""
+        W = np.zeros(self.state_shape()) if out is None else out
+        check(lib.omhd_get_state(self.h, _cptr(W, int(np.prod(self.state_shape())))), "omhd_get_state")
+        return W
+
+    def step_fixed(self, dt, ch):
+        """Advance len(dt) steps with the given dt and ch per step (set_dt
+        replaced by the sequence: the frozen-dt map the adjoint differentiates)."""
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        ch = np.ascontiguousarray(ch, dtype=np.float64)
+        if dt.shape != ch.shape or dt.ndim != 1:
+            raise ValueError("dt and ch: equal-length 1-D sequences")
+        check(lib.omhd_step_fixed(self.h, len(dt), dt.ctypes.data_as(DP), ch.ctypes.data_as(DP)), "omhd_step_fixed")
+        self.t += float(dt.sum())
 
     def layout(self):
         p, plane, pitch = DP(), C.c_int64(), I32()

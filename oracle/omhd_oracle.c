@@ -8,8 +8,8 @@
  * Every routine follows the reference Fortran expression by expression, in the
  * same evaluation order (Fortran evaluates left to right; no FMA contraction:
  * build with -ffp-contract=off), so that its results are bitwise equal to the
- * flang-built reference (checked in tests/test_oracle_vs_ref.py and against the
- * committed fixtures in tests/golden/).
+ * flang-built reference (checked against the committed fixtures in
+ * tests/golden/, which the reference's own routines produced).
  *
  * Arrays are Fortran column-major with the reference's single ghost layer:
  * U(ix,jx,kx,9), V(ix,jx,kx,4), VL/VR/F(ix,jx,kx,9).  2D is kx == 1.
@@ -24,7 +24,10 @@
  *   or_rk_*           flang vectors
  *   or_set_dt         flang vectors
  *   or_bc_*           flang vectors (periodic, KH walls)
- *   or_step2d         OT 200^2 MC 1000-step trace from the flang build
+ *   or_step / or_run  OT 200^2 MC 1000-step dt trace and final-state hash from
+ *                     the flang build; KH walls 100 steps; 3D periodic 4 steps
+ *   or_eta_faces,     closed form of the resistivity profile; the resistive dt
+ *   or_set_dt_resistive  limit 0.5 cfl dx^2 / eta_max (flux_resistive.cuf:2-9)
  *   or_flux_resistive parity unpinned against a reference CPU build (the
  *                     reference has only the CUDA Fortran 3D form); pinned by
  *                     closed forms (uniform B -> no change, linear B -> const J)

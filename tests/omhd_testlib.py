@@ -406,3 +406,31 @@ def orszag_tang(n):
     U[:, :, 7] = 0.0
     U[:, :, 8] = 0.0
     return U, dx
+
+
+class AdjCheck:
+    """Host build of the adjoint building blocks (tests/native/adjcheck.cpp over
+    openmhd_amd/csrc/omhd_adjoint.h): templated forward forms (double and dual
+    numbers) and the hand-written reverse forms.  Built with g++ if missing."""
+
+    def __init__(self):
+        import subprocess
+
+        d = Path(__file__).resolve().parent / "native"
+        so = d / "libadjcheck.so"
+        if not so.exists():
+            subprocess.run(["make", "-s", "-C", str(d)], check=True)
+        self.L = C.CDLL(str(so))
+
+    def __getattr__(self, name):
+        return getattr(self.L, "adjc_" + name)
+
+
+def cptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def soa(U):
+    """Interior of a Fortran U(ix, jx, 9) as W(9, ny, nx), x fastest."""
+    return np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))

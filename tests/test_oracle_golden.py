@@ -171,3 +171,26 @@ def test_resistive_closed_form(oracle):
     for d in (1, 2, 3):
         oracle.L.or_flux_resistive(ptr(F), ptr(U), ptr(eta), dx, ix, jx, kx, d)
     assert np.all(F == 0)
+
+
+def test_resistivity_profile_and_dt_closed_form(oracle):
+    """eta = eta0 + eta1 exp(-(x^2+y^2)/w^2) at x-faces (x + dx/2), y-faces
+    (y + dx/2) and z-faces (cell centre); its peak sits on the face nearest the
+    origin; the resistive dt limit is 0.5 cfl dx^2 / eta_max when it binds
+    (3D_reconnection_gpu/flux_resistive.cuf:2-9)."""
+    import ctypes as C
+
+    dx, ix, jx = 0.25, 12, 10
+    x = -1.5 + dx * np.arange(ix)
+    y = -1.25 + dx * np.arange(jx)
+    out, emax = oracle.eta_faces(x, y, dx, 1e-3, 1e-2, 2.0)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    for d, (ox, oy) in enumerate(((0.5 * dx, 0), (0, 0.5 * dx), (0, 0))):
+        ref = 1e-3 + 1e-2 * np.exp(-((X + ox) ** 2 + (Y + oy) ** 2) / 4.0)
+        np.testing.assert_allclose(out[d], ref, rtol=1e-15, atol=0)
+    assert emax == out.max() and abs(emax - (1e-3 + 1e-2)) < 1e-12  # a face passes through the origin
+    f = oracle.L.or_set_dt_resistive
+    f.restype = C.c_double
+    f.argtypes = [C.c_double] * 4
+    assert f(1.0, 0.1, 0.4, 0.011) == 0.5 * 0.4 * (0.1 * 0.1) / 0.011
+    assert f(1e-4, 0.1, 0.4, 0.011) == 1e-4
