@@ -136,18 +136,26 @@ def bench_ensemble(a):
 
 
 def bench_3d(a):
-    """Config C5: 3D resistive Harris sheet 512x256x256, HLLD + resistive flux
-    + GLM + TVD-RK2, minmod, x/z periodic, y walls."""
+    """Config C5: 3D resistive Harris sheet 512x256x256 (Bx = tanh(y), rho =
+    0.2 + cosh^-2(y)/0.2, eta = 1e-3 + 1e-2 exp(-(x^2+y^2)/4), 1e-3 velocity
+    noise), HLLD + resistive flux + GLM + TVD-RK2, minmod, x/z periodic, y
+    walls: forward zone-updates/s over `steps` steps; then (unless --no-grad)
+    the gradient of the reconnected flux at the end of 100 steps with respect
+    to (eta1, w) by the adjoint with checkpoints every 10 steps, checked by
+    central differences with the frozen dt sequence."""
+    import numpy as np
     import torch
 
-    from openmhd_amd import Solver, problems
+    from openmhd_amd import Adjoint, Solver, problems
 
     torch.cuda.set_device(0)
-    nx, ny, nz, dx = 512, 256, 256, 0.125
+    nx, ny, nz, dx = a.nx3, a.ny3, a.nz3, 0.125
+    eta0, eta1, w = 1e-3, 1e-2, 2.0
     U, origin = problems.harris_sheet_3d(nx, ny, nz, dx)
     s = Solver(nx, ny, dx, nz=nz, cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"),
-               resistivity=(1e-3, 1e-2, 2.0), origin=origin)
+               resistivity=(eta0, eta1, w), origin=origin)
     s.upload(U)
+    W0 = s.get_state()
     del U
     s.step(a.warmup)
     s.set_timing(True)
@@ -157,12 +165,65 @@ def bench_3d(a):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms, n = s.timing()
-    s.close()
     cells = nx * ny * nz
-    print(json.dumps({"workload": "C5 3D resistive Harris 512x256x256", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
-                      "zone_updates_per_s": cells * a.steps / el,
-                      "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
-                      "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}), flush=True)
+    res = {"workload": f"C5 3D resistive Harris {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
+           "zone_updates_per_s": cells * a.steps / el,
+           "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+           "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}
+    log(f"forward: {res['ms_per_step']:.3f} ms/step")
+    if not a.no_grad:
+        nsteps, every = 100, 10
+        j0 = ny // 2  # rows j0-1 and j0 straddle the sheet centre y = 0
+
+        def flux(W):
+            """reconnected flux: half the z-averaged unsigned By flux through y = 0"""
+            byc = 0.5 * (W[6, :, j0 - 1, :] + W[6, :, j0, :])
+            return 0.5 * np.abs(byc).sum() * dx / nz, byc
+
+        s.set_state(W0)
+        adj = Adjoint(s, nsteps, every)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dts, chs = adj.forward()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        WT = s.get_state()
+        phi, byc = flux(WT)
+        lamT = np.zeros_like(WT)
+        g_by = 0.5 * 0.5 * np.sign(byc) * dx / nz
+        lamT[6, :, j0 - 1, :] = g_by
+        lamT[6, :, j0, :] = g_by
+        del WT
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        adj.backward(lamT)
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        g = adj.eta_grad()
+        adj.close()
+        log(f"adjoint: forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, dphi/d(eta0, eta1, w) = {g}")
+        fd = {}
+        for k, name in ((1, "eta1"), (2, "w")):
+            rel = []
+            for eps in (1e-4, 1e-5, 1e-6):
+                L = []
+                for sgn in (1, -1):
+                    prm = [eta0, eta1, w]
+                    prm[k] *= 1 + sgn * eps
+                    s.set_resistivity(*prm)
+                    s.set_state(W0)
+                    s.step_fixed(dts, chs)
+                    L.append(flux(s.get_state())[0])
+                d = (L[0] - L[1]) / (2 * eps * [eta0, eta1, w][k])
+                rel.append(abs(d - g[k]) / abs(g[k]))
+                log(f"d phi / d {name}: adjoint {g[k]:.10e}, central difference (eps {eps:g}) {d:.10e}, rel {rel[-1]:.1e}")
+            fd[name] = rel
+        s.set_resistivity(eta0, eta1, w)
+        res.update({"grad_steps": nsteps, "checkpoint_every": every, "reconnected_flux": phi,
+                    "dphi_deta1": g[1], "dphi_dw": g[2], "fd_rel_err_eps_1e-4_1e-5_1e-6": fd,
+                    "adjoint_forward_s": t_fwd, "adjoint_backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd})
+    s.close()
+    print(json.dumps(res), flush=True)
 
 
 def bench_adjoint(a):
@@ -273,6 +334,10 @@ def main():
     ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d", "adjoint"),
                     help="kh: the headline line; ensemble / adjoint / 3d: configurations C3 / C4 / C5 (own JSON lines)")
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--nx3", type=int, default=512)
+    ap.add_argument("--ny3", type=int, default=256)
+    ap.add_argument("--nz3", type=int, default=256)
+    ap.add_argument("--no-grad", action="store_true", help="3d: forward timing only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()

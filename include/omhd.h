@@ -151,6 +151,10 @@ int omhd_download(omhd_handle *h, double *U, double *V);
  * keep the data on the GPU (the PyTorch binding, the benchmark). */
 int omhd_state_layout(omhd_handle *h, double **dev_state, int64_t *plane_stride, int32_t *row_pitch);
 
+/* Replace the resistivity parameters of a handle created with resistive = 1
+ * (the face arrays and the resistive dt limit follow). */
+int omhd_set_resistivity(omhd_handle *h, double eta0, double eta1, double eta_w);
+
 /* The state as W(9, nz, ny, nx) -- interior cells only, x fastest, slots in
  * the order of param.h:6-15 -- from or to host or device memory (the layout
  * of the PyTorch binding).  omhd_set_state also computes the CFL speed. */
@@ -192,20 +196,24 @@ int omhd_batch_time(omhd_batch *b, double *t);
 int omhd_batch_autotune(omhd_batch *b, int32_t trial_steps, int32_t *rows, int32_t *grid);
 int omhd_batch_destroy(omhd_batch *b);
 
-/* ---- discrete adjoint of the 2D step ------------------------------------- */
+/* ---- discrete adjoint of the step ---------------------------------------- */
 /* No counterpart in the reference.  The reverse-mode derivative of nsteps
- * steps of h (frozen dt/ch sequence) with respect to the initial state:
- * hand-written reverse kernels of u2v, the limiter (the branch the forward
- * pass took), HLLD, glm_ss2 and the RK stages.  omhd_adjoint_forward runs the
- * steps from h's current state (with the CFL dt, recorded into dt_out/ch_out,
- * or with dt_in/ch_in imposed), keeping the state every `every` steps;
- * omhd_adjoint_backward maps lam_T = dL/dU(T) to lam_0 = dL/dU(0), replaying
- * each segment from its checkpoint.  lam_T and lam_0 use the W(9, nz, ny, nx)
- * layout of omhd_set_state, host or device memory.  Periodic 2D grids. */
+ * steps of h (frozen dt/ch sequence) with respect to the initial state, and,
+ * with the resistive flux, to the resistivity parameters: hand-written reverse
+ * kernels of u2v, the limiter (the branch the forward pass took), HLLD, the
+ * resistive flux, the boundary maps, glm_ss2 and the RK stages.
+ * omhd_adjoint_forward runs the steps from h's current state (with the CFL
+ * dt, recorded into dt_out/ch_out, or with dt_in/ch_in imposed), keeping the
+ * state every `every` steps; omhd_adjoint_backward maps lam_T = dL/dU(T) to
+ * lam_0 = dL/dU(0), replaying each segment from its checkpoint.  lam_T and
+ * lam_0 use the W(9, nz, ny, nx) layout of omhd_set_state, host or device
+ * memory.  2D and 3D, periodic and wall boundaries.  After a backward pass,
+ * omhd_adjoint_eta_grad gives dL/d(eta0, eta1, eta_w). */
 typedef struct omhd_adjoint omhd_adjoint;
 int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every, omhd_adjoint **a);
 int omhd_adjoint_forward(omhd_adjoint *a, const double *dt_in, const double *ch_in, double *dt_out, double *ch_out);
 int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, double *lam_0);
+int omhd_adjoint_eta_grad(omhd_adjoint *a, double *grad3);
 int omhd_adjoint_destroy(omhd_adjoint *a);
 
 /* Library version and the gfx target it was built for. */

@@ -103,6 +103,8 @@ def _load() -> C.CDLL:
         "omhd_adjoint_forward": [C.c_void_p, DP, DP, DP, DP],
         "omhd_adjoint_backward": [C.c_void_p, DP, DP],
         "omhd_adjoint_destroy": [C.c_void_p],
+        "omhd_adjoint_eta_grad": [C.c_void_p, DP],
+        "omhd_set_resistivity": [C.c_void_p, C.c_double, C.c_double, C.c_double],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

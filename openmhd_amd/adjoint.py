@@ -1,5 +1,5 @@
-"""Discrete adjoint of the 2D step (libomhd's omhd_adjoint_*) and its
-torch.autograd.Function.
+"""Discrete adjoint of the step (libomhd's omhd_adjoint_*: 2D and 3D, periodic
+and wall boundaries, resistive flux) and its torch.autograd.Function.
 
 The forward pass advances a Solver nsteps steps from W0 keeping the state
 every `every` steps; the backward pass maps dL/dW(T) to dL/dW(0) with the
@@ -61,6 +61,12 @@ class Adjoint:
             out = np.zeros(self.solver.state_shape())
         check(lib.omhd_adjoint_backward(self.a, _cptr(lam_T, n), _cptr(out, n)), "omhd_adjoint_backward")
         return out
+
+    def eta_grad(self):
+        """dL/d(eta0, eta1, eta_w) of the last backward pass (resistive step)."""
+        g = np.zeros(3)
+        check(lib.omhd_adjoint_eta_grad(self.a, g.ctypes.data_as(DP)), "omhd_adjoint_eta_grad")
+        return g
 
     def close(self):
         if getattr(self, "a", None):

@@ -1,25 +1,32 @@
-// omhd_adjoint.hip -- discrete adjoint of the resident 2D step.
+// omhd_adjoint.hip -- discrete adjoint of the resident step (2D and 3D,
+// periodic and wall boundaries, optional resistive flux).
 //
 // The forward map of one step with frozen (dt, ch) (2D_basic/main.f90:80-185):
 //   Ug   = G(U)                       glm_ss2: psi *= f, f = exp(-0.5 dt ch / 0.2)
 //   U*   = Ug + a1 D(Ug)              rk_tvd21 (a1 = dt/dx) | rk_std21 (0.5 dt/dx)
 //   U**  = c0 Ug + c1 U* + a2 D(U*)   rk_tvd22 (0.5, 0.5, 0.5 dt/dx) | rk_std22 (1, 0, dt/dx)
 //   U'   = G(U**)
-// where D(W) = F(i-1) - F(i) + G(j-1) - G(j) is the flux difference from
-// u2v -> limiter -> HLLD.  Its reverse, given lam' = dL/dU':
+// where D(W) = F(i-1) - F(i) + G(j-1) - G(j) [+ H(k-1) - H(k)] is the flux
+// difference from u2v -> limiter -> HLLD (+ the resistive flux).  Its reverse, given lam' = dL/dU':
 //   lam** = G lam'
 //   lam*  = c1 lam** + (dD/dW at U*)^T (a2 lam**)
 //   lamg  = c0 lam** + lam* + (dD/dW at Ug)^T (a1 lam*)
 //   lam   = G lamg
-// Each (dD/dW)^T mu is four launches over the grid:
+// Each (dD/dW)^T mu is 1 + 2 x (number of axes) launches over the grid:
 //   k_adj_prim     primitives of W (u2v)
 //   k_adj_face<x>  per x-face: recompute the two limited face states and the
 //                  HLLD flux, run its reverse on bF = mu(c+1) - mu(c), store the
 //                  adjoints of the face states by owning cell (AL at c, AR at c+1)
 //   k_adj_gather<x> per cell: the limiter reverse gathered from the three cells
 //                  whose slopes read it, into bw
-//   k_adj_face<y>, k_adj_gather<y, finish>: the same along y, then the u2v
-//                  reverse and the RK/GLM combine above, written in place.
+//   k_adj_face<y>, k_adj_gather<y> (and z): the same along y (z); the last
+//                  gather adds the resistive B adjoints, runs the u2v reverse
+//                  and the RK/GLM combine above, written in place.
+// Walls: a wall face's states are mirrors of the inside face state, so its
+// reverse folds both state adjoints onto the inside one; the slopes of the
+// two edge cells read the cell itself through its mirror ghost.
+// Resistive flux: its reverse adds the B adjoints of each face's ten-cell
+// stencil with atomics (stencils overlap) and the eta adjoint per face.
 // Branches (limiter min/sign, HLLD regions) are re-decided from the recomputed
 // values, i.e. the adjoint follows the branch the forward pass took.
 //
@@ -43,10 +50,12 @@ using adj::NV;
 
 struct Geo {
   long long plane, zstride;
-  int pitch, nx, ny, nz;
+  int pitch, nx, ny, nz, nd;
+  int bc[3];               // OMHD_BC_* per axis
+  unsigned slo[3], shi[3]; // slots negated in the low / high mirror ghosts
+  const double *eta;       // resistive: 3 face arrays of (nx+2)*(ny+2), or null
+  double dx;
 };
-
-__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 __device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,17 +67,50 @@ __device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k) {
   return true;
 }
 
-// offset of the cell `s` steps along DIR from (i, j, k), periodic
+__device__ __forceinline__ int map_ax(int c, int n, int bc, unsigned slo, unsigned shi, unsigned &mask) {
+  if (c < 0) {
+    if (bc == OMHD_BC_PERIODIC) return c + n;
+    mask ^= slo;
+    return -1 - c;
+  }
+  if (c >= n) {
+    if (bc == OMHD_BC_PERIODIC) return c - n;
+    mask ^= shi;
+    return 2 * n - 1 - c;
+  }
+  return c;
+}
+
+// interior cell of (i, j, k) (each coordinate at most two cells outside) and
+// the negate mask of its ghost image: periodic wraps, mirrors with sign tables
+struct Cell {
+  long long o;
+  unsigned mask;
+};
+__device__ __forceinline__ Cell cell_at(const Geo &g, int i, int j, int k) {
+  unsigned mk = 0;
+  i = map_ax(i, g.nx, g.bc[0], g.slo[0], g.shi[0], mk);
+  j = map_ax(j, g.ny, g.bc[1], g.slo[1], g.shi[1], mk);
+  if (g.nd == 3) k = map_ax(k, g.nz, g.bc[2], g.slo[2], g.shi[2], mk);
+  return Cell{(long long)k * g.zstride + (long long)j * g.pitch + i, mk};
+}
+
 template <int DIR>
-__device__ __forceinline__ long long nb(const Geo &g, int i, int j, int k, int s) {
-  if (DIR == 0) i = wrap(i + s, g.nx);
-  if (DIR == 1) j = wrap(j + s, g.ny);
-  if (DIR == 2) k = wrap(k + s, g.nz);
-  return (long long)k * g.zstride + (long long)j * g.pitch + i;
+__device__ __forceinline__ Cell nbr(const Geo &g, int i, int j, int k, int s) {
+  return cell_at(g, i + (DIR == 0 ? s : 0), j + (DIR == 1 ? s : 0), k + (DIR == 2 ? s : 0));
+}
+
+__device__ __forceinline__ double flip(double v, unsigned mask, int m) {
+  const long long sbit = (long long)((mask >> m) & 1u) << 63;
+  return __longlong_as_double(__double_as_longlong(v) ^ sbit);
+}
+
+__device__ __forceinline__ double ld(const double *a, const Geo &g, const Cell &c, int m) {
+  return flip(a[m * g.plane + c.o], c.mask, m);
 }
 
 __global__ __launch_bounds__(256) void k_adj_prim(const Geo g, const double *__restrict__ U, double psi_f,
-                                                  double *__restrict__ w) {
+                                                  double *__restrict__ w, double *__restrict__ R) {
   int i, j, k;
   if (!cell_of(g, i, j, k)) return;
   const long long o = (long long)k * g.zstride + (long long)j * g.pitch + i;
@@ -79,37 +121,169 @@ __global__ __launch_bounds__(256) void k_adj_prim(const Geo g, const double *__r
   adj::prim_fwd<double>(u, p);
 #pragma unroll
   for (int m = 0; m < NV; m++) w[m * g.plane + o] = p[m];
+  if (R) {  // resistive B adjoints, accumulated by the face kernels
+#pragma unroll
+    for (int m = 0; m < 3; m++) R[m * g.plane + o] = 0.0;
+  }
 }
 
-// mu = mscale * M, slot 8 also times mpsi
+// Reverse of the resistive flux at the face with left cell (i,j,k) along DIR
+// (3D_reconnection_gpu/flux_resistive.cuf:86-135): the B adjoints go to the
+// ten stencil cells (atomically: stencils of neighbouring faces overlap), the
+// eta adjoint to the face's entry of geta.  J1 pairs with slot q1 in the
+// energy flux and feeds slot q1's flux with +eta J1; J2 with q2 and -eta J2.
+template <int DIR>
+__device__ void resistive_vjp(const Geo &g, const double *__restrict__ w, int i, int j, int k, const double bF[NV],
+                              double *__restrict__ R, double *__restrict__ geta) {
+  constexpr int q1 = DIR == 0 ? 7 : (DIR == 1 ? 5 : 6);  // x: bz, y: bx, z: by
+  constexpr int q2 = DIR == 0 ? 6 : (DIR == 1 ? 7 : 5);  // x: by, y: bz, z: bx
+  constexpr int t1 = DIR == 0 ? 5 : (DIR == 1 ? 6 : 7);  // slot of J1's transverse differences
+  constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+  // J1's 0.25-terms vary along axis A1, J2's along A2 (x: z and y; y: x and z; z: y and x)
+  constexpr int A1 = DIR == 0 ? 2 : (DIR == 1 ? 0 : 1);
+  constexpr int A2 = DIR == 0 ? 1 : (DIR == 1 ? 2 : 0);
+  const int ix = g.nx + 2, jx = g.ny + 2;
+  const long long ei = (long long)DIR * ix * jx + (long long)(i + 1) + (long long)ix * (j + 1);
+  const double e = g.eta[ei] / g.dx;
+  const Cell c0 = cell_at(g, i, j, k), c1 = cell_at(g, i + di, j + dj, k + dk);
+  auto sh = [&](int a, int s, int base_i, int base_j, int base_k) {
+    return cell_at(g, base_i + (a == 0 ? s : 0), base_j + (a == 1 ? s : 0), base_k + (a == 2 ? s : 0));
+  };
+  const Cell p1a = sh(A1, 1, i, j, k), p1b = sh(A1, 1, i + di, j + dj, k + dk);
+  const Cell m1a = sh(A1, -1, i, j, k), m1b = sh(A1, -1, i + di, j + dj, k + dk);
+  const Cell p2a = sh(A2, 1, i, j, k), p2b = sh(A2, 1, i + di, j + dj, k + dk);
+  const Cell m2a = sh(A2, -1, i, j, k), m2b = sh(A2, -1, i + di, j + dj, k + dk);
+  // J1 = -(q1(c+1) - q1(c)) + 0.25 (t1(+A1) terms - t1(-A1) terms)  [x: JyS, y: JzS, z: JxS]
+  // J2 =  (q2(c+1) - q2(c)) - 0.25 (t2(+A2) terms - t2(-A2) terms)  with t2 = t1's slot
+  const double J1 = -(ld(w, g, c1, q1) - ld(w, g, c0, q1)) +
+                    0.25 * (ld(w, g, p1a, t1) + ld(w, g, p1b, t1) - ld(w, g, m1a, t1) - ld(w, g, m1b, t1));
+  const double J2 = (ld(w, g, c1, q2) - ld(w, g, c0, q2)) -
+                    0.25 * (ld(w, g, p2a, t1) + ld(w, g, p2b, t1) - ld(w, g, m2a, t1) - ld(w, g, m2b, t1));
+  const double S1 = ld(w, g, c0, q1) + ld(w, g, c1, q1), S2 = ld(w, g, c0, q2) + ld(w, g, c1, q2);
+  // F[3] += 0.5 e (J1 S1 - J2 S2); F[q1] += e J1; F[q2] -= e J2
+  const double bJ1 = e * (0.5 * bF[3] * S1 + bF[q1]);
+  const double bJ2 = e * (-0.5 * bF[3] * S2 - bF[q2]);
+  const double bS1 = 0.5 * e * bF[3] * J1, bS2 = -0.5 * e * bF[3] * J2;
+  auto add = [&](const Cell &c, int m, double v) {
+    atomicAdd(&R[(m - 5) * g.plane + c.o], flip(v, c.mask, m));
+  };
+  add(c0, q1, bJ1 + bS1);
+  add(c1, q1, -bJ1 + bS1);
+  add(p1a, t1, 0.25 * bJ1);
+  add(p1b, t1, 0.25 * bJ1);
+  add(m1a, t1, -0.25 * bJ1);
+  add(m1b, t1, -0.25 * bJ1);
+  add(c0, q2, -bJ2 + bS2);
+  add(c1, q2, bJ2 + bS2);
+  add(p2a, t1, -0.25 * bJ2);
+  add(p2b, t1, -0.25 * bJ2);
+  add(m2a, t1, 0.25 * bJ2);
+  add(m2b, t1, 0.25 * bJ2);
+  atomicAdd(&geta[ei], (0.5 * bF[3] * (J1 * S1 - J2 * S2) + bF[q1] * J1 - bF[q2] * J2) / g.dx);
+}
+
+// The face-state adjoints of one face.  kind 0: interior/periodic face
+// (c, c+1); 1: high wall face (c = n-1, right state = mirror of the left);
+// 2: low wall face (-1, 0) (left state = mirror of the right).  mu =
+// mscale * M (slot 8 also times mpsi).  Stored by owning cell: AL(c) for the
+// left state of (c, c+1), AR(c) for the right state of (c-1, c); a wall
+// face's two states both belong to its interior cell.
 template <int DIR, int LIM>
-__global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__restrict__ w,
-                                                  const double *__restrict__ M, double mscale, double mpsi, double ch,
-                                                  double *__restrict__ AL, double *__restrict__ AR) {
-  int i, j, k;
-  if (!cell_of(g, i, j, k)) return;
-  const long long om = nb<DIR>(g, i, j, k, -1), o0 = nb<DIR>(g, i, j, k, 0), o1 = nb<DIR>(g, i, j, k, 1),
-                  o2 = nb<DIR>(g, i, j, k, 2);
+__device__ __forceinline__ void face_vjp(const Geo &g, const double *__restrict__ w, const double *__restrict__ M,
+                                         double mscale, double mpsi, double ch, double *__restrict__ AL,
+                                         double *__restrict__ AR, int i, int j, int k, int kind) {
+  const Cell cm = nbr<DIR>(g, i, j, k, -1), c0 = nbr<DIR>(g, i, j, k, 0), c1 = nbr<DIR>(g, i, j, k, 1);
   double VL[NV], VR[NV], bF[NV], bL[NV], bR[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    const double *a = w + m * g.plane;
-    const double am = a[om], a0 = a[o0], a1 = a[o1], a2 = a[o2];
-    double gR, gL, hR, hL;
+    const double am = ld(w, g, cm, m), a0 = ld(w, g, c0, m), a1 = ld(w, g, c1, m);
+    double gR, gL;
     adj::lim_fwd<double>(LIM, am, a0, a1, gR, gL);
-    adj::lim_fwd<double>(LIM, a0, a1, a2, hR, hL);
-    VL[m] = gL;
-    VR[m] = hR;
     const double s = m == 8 ? mscale * mpsi : mscale;
-    bF[m] = s * (M[m * g.plane + o1] - M[m * g.plane + o0]);  // F enters D(c) with -1, D(c+1) with +1
+    const double mu0 = M[m * g.plane + c0.o];
+    if (kind == 0) {
+      const Cell c2 = nbr<DIR>(g, i, j, k, 2);
+      double hR, hL;
+      adj::lim_fwd<double>(LIM, a0, a1, ld(w, g, c2, m), hR, hL);
+      VL[m] = gL;
+      VR[m] = hR;
+      bF[m] = s * (M[m * g.plane + c1.o] - mu0);  // F enters D(c) with -1, D(c+1) with +1
+    } else if (kind == 1) {
+      VL[m] = gL;
+      VR[m] = flip(gL, g.shi[DIR], m);
+      bF[m] = -(s * mu0);
+    } else {
+      VR[m] = gR;
+      VL[m] = flip(gR, g.slo[DIR], m);
+      bF[m] = s * mu0;
+    }
     bL[m] = 0.0;
     bR[m] = 0.0;
   }
   adj::hlld_vjp<DIR>(VL, VR, ch, bF, bL, bR);
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    AL[m * g.plane + o0] = bL[m];
-    AR[m * g.plane + o1] = bR[m];
+    if (kind == 0) {
+      AL[m * g.plane + c0.o] = bL[m];
+      AR[m * g.plane + c1.o] = bR[m];
+    } else if (kind == 1) {
+      AL[m * g.plane + c0.o] = bL[m] + flip(bR[m], g.shi[DIR], m);
+    } else {
+      AR[m * g.plane + c0.o] = bR[m] + flip(bL[m], g.slo[DIR], m);
+    }
+  }
+}
+
+// one thread per cell c along DIR: the face (c, c+1), or the high wall face
+template <int DIR, int LIM>
+__global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__restrict__ w,
+                                                  const double *__restrict__ M, double mscale, double mpsi, double ch,
+                                                  double *__restrict__ AL, double *__restrict__ AR) {
+  int i, j, k;
+  if (!cell_of(g, i, j, k)) return;
+  const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  const bool high = g.bc[DIR] != OMHD_BC_PERIODIC && c == n - 1;
+  face_vjp<DIR, LIM>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, high ? 1 : 0);
+}
+
+// walls only: one thread per cell of the plane c = 0, the low wall face
+template <int DIR, int LIM>
+__global__ __launch_bounds__(128) void k_adj_face_lo(const Geo g, const double *__restrict__ w,
+                                                     const double *__restrict__ M, double mscale, double mpsi,
+                                                     double ch, double *__restrict__ AL, double *__restrict__ AR) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int na = DIR == 0 ? g.ny : g.nx, nb = DIR == 2 ? g.ny : g.nz;  // the two transverse extents
+  if (t >= (long long)na * nb) return;
+  const int u = (int)(t % na), v = (int)(t / na);
+  const int i = DIR == 0 ? 0 : u, j = DIR == 0 ? u : (DIR == 1 ? 0 : v), k = DIR == 2 ? 0 : v;
+  face_vjp<DIR, LIM>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, 2);
+}
+
+// the resistive flux's reverse at every face along DIR: one thread per cell
+// c (the face (c, c+1), or the high wall face), plus the low wall face for
+// c = 0 at a wall; bF is rebuilt from mu as the face kernels build it
+template <int DIR>
+__global__ __launch_bounds__(256) void k_adj_res(const Geo g, const double *__restrict__ w,
+                                                 const double *__restrict__ M, double mscale, double mpsi,
+                                                 double *__restrict__ R, double *__restrict__ geta) {
+  int i, j, k;
+  if (!cell_of(g, i, j, k)) return;
+  const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  const bool wall = g.bc[DIR] != OMHD_BC_PERIODIC;
+  const Cell c0 = nbr<DIR>(g, i, j, k, 0), c1 = nbr<DIR>(g, i, j, k, 1);
+  double bF[NV];
+  for (int f = 0; f < ((wall && c == 0) ? 2 : 1); f++) {
+    const int kind = f == 1 ? 2 : ((wall && c == n - 1) ? 1 : 0);
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const double s = m == 8 ? mscale * mpsi : mscale;
+      const double mu0 = M[m * g.plane + c0.o];
+      bF[m] = kind == 0 ? s * (M[m * g.plane + c1.o] - mu0) : (kind == 1 ? -(s * mu0) : s * mu0);
+    }
+    if (kind == 2) resistive_vjp<DIR>(g, w, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2), bF, R, geta);
+    else resistive_vjp<DIR>(g, w, i, j, k, bF, R, geta);
   }
 }
 
@@ -122,36 +296,44 @@ struct Combine {
   double *out;
 };
 
-// MODE 0: bw = contribution; 1: bw += contribution; 2: finish (u2v reverse and
-// the RK/GLM combine into c.out)
+// MODE 0: bw = contribution; 1: bw += contribution; 2: finish (resistive B
+// adjoints, u2v reverse and the RK/GLM combine into c.out)
 template <int DIR, int LIM, int MODE>
 __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *__restrict__ w,
                                                     const double *__restrict__ AL, const double *__restrict__ AR,
-                                                    double *__restrict__ bw, const Combine c) {
+                                                    double *__restrict__ bw, const double *__restrict__ R,
+                                                    const Combine c) {
   int i, j, k;
   if (!cell_of(g, i, j, k)) return;
-  const long long o0 = nb<DIR>(g, i, j, k, 0);
-  long long o[5];
+  const int cc = DIR == 0 ? i : (DIR == 1 ? j : k);
+  const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  const bool wall = g.bc[DIR] != OMHD_BC_PERIODIC;
+  Cell o[5];
 #pragma unroll
-  for (int s = 0; s < 5; s++) o[s] = nb<DIR>(g, i, j, k, s - 2);
+  for (int s = 0; s < 5; s++) o[s] = nbr<DIR>(g, i, j, k, s - 2);
+  const long long o0 = o[2].o;
   double b[NV], wc[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    const double *a = w + m * g.plane;
     double v[5];
 #pragma unroll
-    for (int s = 0; s < 5; s++) v[s] = a[o[s]];
+    for (int s = 0; s < 5; s++) v[s] = ld(w, g, o[s], m);
     wc[m] = v[2];
     double acc = 0.0;
-    // the slope at cell q = c + s (s = -1, 0, 1) reads this cell as its
-    // (wp, w0, wm) neighbour: coefficient index 1 - s
+    // the slopes at cells q = c + s (s = -1, 0, 1) read this cell as their
+    // (wp, w0, wm) neighbour: coefficient index 1 - s; at a wall, q = 0 and
+    // q = n - 1 also read it through their mirror ghost
 #pragma unroll
     for (int s = -1; s <= 1; s++) {
+      const int q = cc + s;
+      if (wall && (q < 0 || q > n - 1)) continue;
       double cR[3], cL[3];
       adj::lim_coef(LIM, v[s + 1], v[s + 2], v[s + 3], cR, cL);
-      const double l = AL[m * g.plane + o[s + 2]], r = AR[m * g.plane + o[s + 2]];
+      const double l = AL[m * g.plane + o[s + 2].o], r = AR[m * g.plane + o[s + 2].o];
       if (s == 0) acc += l + r;  // gL = w + gradL, gR = w - gradR
       acc += cL[1 - s] * l - cR[1 - s] * r;
+      if (wall && s == 0 && q == 0) acc += flip(cL[0] * l - cR[0] * r, g.slo[DIR], m);
+      if (wall && s == 0 && q == n - 1) acc += flip(cL[2] * l - cR[2] * r, g.shi[DIR], m);
     }
     b[m] = acc;
   }
@@ -166,6 +348,10 @@ __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *_
 #pragma unroll
     for (int m = 0; m < NV; m++) bw[m * g.plane + o0] = b[m];
     return;
+  }
+  if (R) {
+#pragma unroll
+    for (int m = 5; m < 8; m++) b[m] += R[(m - 5) * g.plane + o0];
   }
   double nu[NV];
 #pragma unroll
@@ -192,18 +378,28 @@ __global__ void k_lam_copy(double *__restrict__ dst, const double *__restrict__ 
 }
 
 typedef void (*face_fn)(const Geo, const double *, const double *, double, double, double, double *, double *);
-typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, double *, const Combine);
+typedef void (*res_fn)(const Geo, const double *, const double *, double, double, double *, double *);
+typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, double *, const double *,
+                          const Combine);
 
-template <int DIR>
+template <int DIR, int LO>
 face_fn face_kernel(int lim) {
   switch (lim) {
-    case 0: return k_adj_face<DIR, 0>;
-    case 1: return k_adj_face<DIR, 1>;
-    case 2: return k_adj_face<DIR, 2>;
-    case 3: return k_adj_face<DIR, 3>;
-    default: return k_adj_face<DIR, 4>;
+    case 0: return LO ? k_adj_face_lo<DIR, 0> : k_adj_face<DIR, 0>;
+    case 1: return LO ? k_adj_face_lo<DIR, 1> : k_adj_face<DIR, 1>;
+    case 2: return LO ? k_adj_face_lo<DIR, 2> : k_adj_face<DIR, 2>;
+    case 3: return LO ? k_adj_face_lo<DIR, 3> : k_adj_face<DIR, 3>;
+    default: return LO ? k_adj_face_lo<DIR, 4> : k_adj_face<DIR, 4>;
   }
 }
+
+face_fn face_kernel(int dir, int lo, int lim) {
+  if (dir == 0) return lo ? face_kernel<0, 1>(lim) : face_kernel<0, 0>(lim);
+  if (dir == 1) return lo ? face_kernel<1, 1>(lim) : face_kernel<1, 0>(lim);
+  return lo ? face_kernel<2, 1>(lim) : face_kernel<2, 0>(lim);
+}
+
+res_fn res_kernel(int dir) { return dir == 0 ? k_adj_res<0> : (dir == 1 ? k_adj_res<1> : k_adj_res<2>); }
 
 template <int DIR, int MODE>
 gather_fn gather_kernel(int lim) {
@@ -214,6 +410,13 @@ gather_fn gather_kernel(int lim) {
     case 3: return k_adj_gather<DIR, 3, MODE>;
     default: return k_adj_gather<DIR, 4, MODE>;
   }
+}
+
+// x writes bw, y adds (3D) or finishes (2D), z finishes
+gather_fn gather_kernel(int dir, int nd, int lim) {
+  if (dir == 0) return gather_kernel<0, 0>(lim);
+  if (dir == 1) return nd == 3 ? gather_kernel<1, 1>(lim) : gather_kernel<1, 2>(lim);
+  return gather_kernel<2, 2>(lim);
 }
 
 }  // namespace
@@ -229,13 +432,17 @@ struct omhd_adjoint {
   double *ckpt = nullptr;      // nseg states: U at steps 0, every, 2 every, ...
   double *tape = nullptr;      // 2 * every states: (U^n, U*^n) of one segment
   double *work = nullptr;      // w, AL, AR, bw, lam, lam*
-  bool have_forward = false;
+  double *R = nullptr;         // resistive: B adjoints (3 planes)
+  double *geta = nullptr;      // resistive: adjoint of each face's eta, 3 (nx+2)(ny+2)
+  std::vector<double> geta_host;
+  bool have_forward = false, have_backward = false;
   double *w() { return work; }
   double *AL() { return work + state; }
   double *AR() { return work + 2 * state; }
   double *bw() { return work + 3 * state; }
   double *lam() { return work + 4 * state; }
   double *lams() { return work + 5 * state; }
+  size_t neta() const { return (size_t)3 * (h->cfg.nx + 2) * (h->cfg.ny + 2); }
 };
 
 static Geo geo_of(const omhd_handle *h) {
@@ -246,16 +453,20 @@ static Geo geo_of(const omhd_handle *h) {
   g.nx = h->cfg.nx;
   g.ny = h->cfg.ny;
   g.nz = h->cfg.nz;
+  g.nd = h->nd;
+  for (int a = 0; a < 3; a++) {
+    g.bc[a] = h->faces.bc[a];
+    g.slo[a] = h->faces.slo[a];
+    g.shi[a] = h->faces.shi[a];
+  }
+  g.eta = h->eta;
+  g.dx = h->cfg.dx;
   return g;
 }
 
 extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every, omhd_adjoint **out) {
   if (!h || !out) return set_error(OMHD_ERR_ARG, "null argument");
   if (nsteps < 1 || every < 1) return set_error(OMHD_ERR_ARG, "need nsteps >= 1 and every >= 1", nsteps, every);
-  if (h->nd != 2) return set_error(OMHD_ERR_ARG, "the adjoint is built for the 2D step");
-  for (int a = 0; a < 2; a++)
-    if (h->faces.bc[a] != OMHD_BC_PERIODIC)
-      return set_error(OMHD_ERR_ARG, "the adjoint is built for periodic boundaries", a);
   if (every > nsteps) every = nsteps;
   omhd_adjoint *a = new omhd_adjoint();
   a->h = h;
@@ -268,6 +479,8 @@ extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every
   hipError_t e = hipMalloc((void **)&a->ckpt, a->state * a->nseg * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&a->tape, a->state * 2 * every * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&a->work, a->state * 6 * sizeof(double));
+  if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->R, (size_t)h->plane * 3 * sizeof(double));
+  if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->geta, a->neta() * sizeof(double));
   if (e != hipSuccess) {
     omhd_adjoint_destroy(a);
     return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
@@ -279,9 +492,8 @@ extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every
 extern "C" int omhd_adjoint_destroy(omhd_adjoint *a) {
   if (!a) return ok();
   if (a->h && a->h->stream) (void)hipStreamSynchronize(a->h->stream);
-  if (a->ckpt) (void)hipFree(a->ckpt);
-  if (a->tape) (void)hipFree(a->tape);
-  if (a->work) (void)hipFree(a->work);
+  for (double *p : {a->ckpt, a->tape, a->work, a->R, a->geta})
+    if (p) (void)hipFree(p);
   delete a;
   return ok();
 }
@@ -292,7 +504,7 @@ extern "C" int omhd_adjoint_forward(omhd_adjoint *a, const double *dt_in, const 
   omhd_handle *h = a->h;
   if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
   if ((dt_in == nullptr) != (ch_in == nullptr)) return set_error(OMHD_ERR_ARG, "give both dt and ch, or neither");
-  a->have_forward = false;
+  a->have_forward = a->have_backward = false;
   for (int s = 0; s < a->nseg; s++) {
     const int n0 = s * a->every, cnt = std::min(a->every, a->nsteps - n0);
     OMHD_HIP(hipMemcpyAsync(a->ckpt + a->state * s, h->U, a->state * sizeof(double), hipMemcpyDeviceToDevice,
@@ -313,29 +525,37 @@ extern "C" int omhd_adjoint_forward(omhd_adjoint *a, const double *dt_in, const 
   return ok();
 }
 
-// nu-combine of one (dD/dW)^T mu at state W (psi of W scaled by psi_f)
+// (dD/dW at W)^T mu, mu = mscale * M (slot 8 also times mpsi), combined into
+// fin.out (psi of W scaled by psi_f on load)
 static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const double *M, double mscale, double mpsi,
                       double ch, const Combine &fin) {
   omhd_handle *h = a->h;
   const Geo g = geo_of(h);
-  const int lim = h->cfg.lm_type;
+  const int lim = h->cfg.lm_type, res = h->cfg.resistive ? 1 : 0;
   const size_t n = (size_t)g.nx * g.ny * g.nz;
   const unsigned g256 = (unsigned)((n + 255) / 256), g128 = (unsigned)((n + 127) / 128);
-  k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w());
+  k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w(), res ? a->R : nullptr);
   OMHD_HIP(hipGetLastError());
-  Combine none{};
-  hipLaunchKernelGGL(face_kernel<0>(lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch, a->AL(),
-                     a->AR());
-  OMHD_HIP(hipGetLastError());
-  hipLaunchKernelGGL((gather_kernel<0, 0>(lim)), dim3(g256), dim3(256), 0, h->stream, g, a->w(), a->AL(), a->AR(),
-                     a->bw(), none);
-  OMHD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(face_kernel<1>(lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch, a->AL(),
-                     a->AR());
-  OMHD_HIP(hipGetLastError());
-  hipLaunchKernelGGL((gather_kernel<1, 2>(lim)), dim3(g256), dim3(256), 0, h->stream, g, a->w(), a->AL(), a->AR(),
-                     a->bw(), fin);
-  OMHD_HIP(hipGetLastError());
+  for (int dir = 0; dir < g.nd; dir++) {
+    hipLaunchKernelGGL(face_kernel(dir, 0, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch,
+                       a->AL(), a->AR());
+    OMHD_HIP(hipGetLastError());
+    if (g.bc[dir] != OMHD_BC_PERIODIC) {
+      const long long nt = (long long)n / (dir == 0 ? g.nx : (dir == 1 ? g.ny : g.nz));
+      hipLaunchKernelGGL(face_kernel(dir, 1, lim), dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, h->stream, g,
+                         a->w(), M, mscale, mpsi, ch, a->AL(), a->AR());
+      OMHD_HIP(hipGetLastError());
+    }
+    if (res) {
+      hipLaunchKernelGGL(res_kernel(dir), dim3(g256), dim3(256), 0, h->stream, g, a->w(), M, mscale, mpsi, a->R,
+                         a->geta);
+      OMHD_HIP(hipGetLastError());
+    }
+    const bool last = dir == g.nd - 1;
+    hipLaunchKernelGGL(gather_kernel(dir, g.nd, lim), dim3(g256), dim3(256), 0, h->stream, g, a->w(), a->AL(), a->AR(),
+                       a->bw(), res ? a->R : nullptr, last ? fin : Combine{});
+    OMHD_HIP(hipGetLastError());
+  }
   return OMHD_OK;
 }
 
@@ -353,6 +573,7 @@ extern "C" int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, doubl
   OMHD_HIP(hipGetLastError());
   in.release(h->stream);
   OMHD_HIP(hipMemsetAsync(h->red, 0, 4 * sizeof(unsigned long long), h->stream));
+  if (a->geta) OMHD_HIP(hipMemsetAsync(a->geta, 0, a->neta() * sizeof(double), h->stream));
   const bool tvd = c.time_type == 0;
   for (int s = a->nseg - 1; s >= 0; s--) {
     const int n0 = s * a->every, cnt = std::min(a->every, a->nsteps - n0);
@@ -389,5 +610,42 @@ extern "C" int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, doubl
   int err[4];
   memcpy(err, &h->red_host[1], sizeof(err));
   if (err[0]) return set_error(err[0], "invalid state while replaying the forward pass", err[1], err[2], err[3]);
+  if (a->geta) {
+    a->geta_host.resize(a->neta());
+    OMHD_HIP(hipMemcpy(a->geta_host.data(), a->geta, a->neta() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  a->have_backward = true;
+  return ok();
+}
+
+extern "C" int omhd_adjoint_eta_grad(omhd_adjoint *a, double *grad) {
+  if (!a || !grad) return set_error(OMHD_ERR_ARG, "null argument");
+  if (!a->h->cfg.resistive) return set_error(OMHD_ERR_ARG, "the step has no resistivity");
+  if (!a->have_backward) return set_error(OMHD_ERR_ARG, "omhd_adjoint_backward first");
+  const omhd_config &c = a->h->cfg;
+  // eta = eta0 + eta1 exp(-r^2/w^2) at the face centres, placed as the
+  // forward places them (cell centres x0 - dx/2 + i dx, accumulated)
+  const int ix = c.nx + 2, jx = c.ny + 2;
+  std::vector<double> x(ix), y(jx);
+  x[0] = c.x0 - c.dx / 2;
+  for (int i = 1; i < ix; i++) x[i] = x[i - 1] + c.dx;
+  y[0] = c.y0 - c.dx / 2;
+  for (int j = 1; j < jx; j++) y[j] = y[j - 1] + c.dx;
+  double g0 = 0, g1 = 0, gw = 0;
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < jx; j++)
+      for (int i = 0; i < ix; i++) {
+        const double ge = a->geta_host[(size_t)d * ix * jx + (size_t)i + (size_t)ix * j];
+        if (ge == 0.0) continue;
+        const double xx = x[i] + (d == 0 ? 0.5 * c.dx : 0.0), yy = y[j] + (d == 1 ? 0.5 * c.dx : 0.0);
+        const double r2 = xx * xx + yy * yy, w2 = c.eta_w * c.eta_w;
+        const double ex = exp(-r2 / w2);
+        g0 += ge;
+        g1 += ge * ex;
+        gw += ge * c.eta1 * ex * 2.0 * r2 / (w2 * c.eta_w);
+      }
+  grad[0] = g0;
+  grad[1] = g1;
+  grad[2] = gw;
   return ok();
 }

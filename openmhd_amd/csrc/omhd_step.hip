@@ -629,6 +629,20 @@ extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
   return ok();
 }
 
+extern "C" int omhd_set_resistivity(omhd_handle *h, double eta0, double eta1, double eta_w) {
+  if (!h) return set_error(OMHD_ERR_ARG, "null handle");
+  if (!h->cfg.resistive) return set_error(OMHD_ERR_ARG, "the handle was created without resistivity");
+  if (!(eta_w > 0)) return set_error(OMHD_ERR_ARG, "eta_w must be positive");
+  h->cfg.eta0 = eta0;
+  h->cfg.eta1 = eta1;
+  h->cfg.eta_w = eta_w;
+  std::vector<double> eh;
+  h->eta_max = eta_faces(h->cfg, eh);
+  OMHD_HIP(hipMemcpyAsync(h->eta, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  OMHD_HIP(hipStreamSynchronize(h->stream));
+  return ok();
+}
+
 extern "C" int omhd_destroy(omhd_handle *h) {
   if (!h) return ok();
   if (h->stream) (void)hipStreamSynchronize(h->stream);

@@ -132,6 +132,10 @@ class Solver:
         check(lib.omhd_step_fixed(self.h, len(dt), dt.ctypes.data_as(DP), ch.ctypes.data_as(DP)), "omhd_step_fixed")
         self.t += float(dt.sum())
 
+    def set_resistivity(self, eta0, eta1, w):
+        """New resistivity parameters (handles created with resistivity=...)."""
+        check(lib.omhd_set_resistivity(self.h, eta0, eta1, w), "omhd_set_resistivity")
+
     def layout(self):
         p, plane, pitch = DP(), C.c_int64(), I32()
         check(lib.omhd_state_layout(self.h, C.byref(p), C.byref(plane), C.byref(pitch)), "omhd_state_layout")
