@@ -153,7 +153,7 @@ def bench_3d(a):
     eta0, eta1, w = 1e-3, 1e-2, 2.0
     U, origin = problems.harris_sheet_3d(nx, ny, nz, dx)
     s = Solver(nx, ny, dx, nz=nz, cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"),
-               resistivity=(eta0, eta1, w), origin=origin)
+               resistivity=None if a.no_res else (eta0, eta1, w), origin=origin)
     s.upload(U)
     W0 = s.get_state()
     del U
@@ -166,12 +166,12 @@ def bench_3d(a):
     el = time.perf_counter() - t0
     ms, n = s.timing()
     cells = nx * ny * nz
-    res = {"workload": f"C5 3D resistive Harris {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
+    res = {"workload": f"C5 3D {'ideal' if a.no_res else 'resistive'} Harris {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
            "zone_updates_per_s": cells * a.steps / el,
            "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
            "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}
     log(f"forward: {res['ms_per_step']:.3f} ms/step")
-    if not a.no_grad:
+    if not a.no_grad and not a.no_res:
         nsteps, every = 100, 10
         j0 = ny // 2  # rows j0-1 and j0 straddle the sheet centre y = 0
 
@@ -347,6 +347,7 @@ def main():
     ap.add_argument("--nz3", type=int, default=256)
     ap.add_argument("--no-grad", action="store_true", help="3d: forward timing only")
     ap.add_argument("--no-fd", action="store_true", help="3d: skip the finite-difference check of the gradient")
+    ap.add_argument("--no-res", action="store_true", help="3d: ideal MHD (no resistive flux), forward only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()

@@ -377,9 +377,10 @@ OMHD_HD void hlld_side_vjp(const double s[NV], const double bU[4], const double 
     const double A = f1 + B2;
     const double q = A * A - f1 * f2;
     const double sq = sqrt(q > 0 ? q : 0.0);
-    const double a = (A + sq) / (2 * ro);
-    const double bN = ba / (2 * ro);
-    bro -= ba * a / ro;
+    const double ir = 1.0 / ro;  // reverse-only arithmetic: no correctly rounded quotients needed
+    const double a = (A + sq) * (0.5 * ir);
+    const double bN = 0.5 * ba * ir;
+    bro -= ba * a * ir;
     double bA = bN;
     const double bq = (q > 0 && sq > 0) ? bN / (2 * sq) : 0.0;
     bA += 2 * A * bq;
@@ -528,8 +529,9 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
     const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
     const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
     const double hb = tabs(Uh[bn]);
-    const double aVL = hb / sqrt(roL);
-    const double aVR = hb / sqrt(roR);
+    const double sroL = sqrt(roL), sroR = sqrt(roR);
+    const double aVL = hb / sroL;
+    const double aVR = hb / sroR;
     double bUh[NV];
     for (int m = 0; m < NV; m++) bUh[m] = 0.0;
     double biw = 0, baM = 0, bpt = 0, broL = 0, broR = 0, baVL = 0, baVR = 0;
@@ -575,8 +577,8 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
       double &bvBS = useL ? bvBL : bvBR;
       const double P = vBS - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2];
       const double Q = (aS - S[vn]) * US[EN] - ptS * S[vn] + pt * aM + Uh[bn] * P;
-      const double dS = aS - aM;
-      const double enS = Q / dS;
+      const double idS = 1.0 / (aS - aM);
+      const double enS = Q * idS;
       // F[mn] = aS (roS aM - US[mn]) + FS[mn]
       baS += (roS * aM - US[mn]) * bF[mn];
       broS += aS * aM * bF[mn];
@@ -604,8 +606,8 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
       bFS[RO] += bF[RO];
       // enS = Q / (aS - aM)
       {
-        const double bQ = benS / dS;
-        const double bd = -benS * enS / dS;
+        const double bQ = benS * idS;
+        const double bd = -bQ * enS;
         baS += bd;
         baM -= bd;
         baS += US[EN] * bQ;
@@ -625,9 +627,10 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
         bUh[bt2] -= at2 * bP;
       }
       // at = Uh[mt] / Uh[RO]
-      bUh[mt1] += bat1 / Uh[RO];
-      bUh[mt2] += bat2 / Uh[RO];
-      bUh[RO] -= (at1 * bat1 + at2 * bat2) / Uh[RO];
+      const double iro = 1.0 / Uh[RO];
+      bUh[mt1] += bat1 * iro;
+      bUh[mt2] += bat2 * iro;
+      bUh[RO] -= (at1 * bat1 + at2 * bat2) * iro;
     } else {
       // ---- HLLD (flux_solver.f90:482-572) ----
       double UL1[NV], UR1[NV];
@@ -698,7 +701,7 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
         // central states (flux_solver.f90:533-572)
         double U2[NV], bU2[NV];
         for (int m = 0; m < NV; m++) bU2[m] = 0.0;
-        const double roLs = sqrt(roL), roRs = sqrt(roR);
+        const double roLs = sroL, roRs = sroR;
         const double fc = 1.0 / (roLs + roRs);
         const double sB = csign(1.0, Uh[bn]);
         const double Na1 = roLs * vt1L + roRs * vt1R + (UR1[bt1] - UL1[bt1]) * sB;
@@ -813,8 +816,8 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
           const double bs = -fc * fc * bfc;
           broLs += bs;
           broRs += bs;
-          broL += broLs / (2 * roLs);
-          broR += broRs / (2 * roRs);
+          broL += broLs * 0.5 * sroL / roL;  // d sqrt(ro) = d ro / (2 sqrt(ro))
+          broR += broRs * 0.5 * sroR / roR;
         }
       }
       // aL1 = aM - aVL, aR1 = aM + aVR
@@ -849,9 +852,8 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
         baM += roS * bUS1[mn];
         // US1[EN] = QS / (aS - aM)
         {
-          const double dS = aS - aM;
-          const double bQ = bUS1[EN] / dS;
-          const double bd = -bUS1[EN] * US1[EN] / dS;
+          const double bQ = bUS1[EN] / (aS - aM);
+          const double bd = -bQ * US1[EN];
           baS += bd;
           baM -= bd;
           baS += US[EN] * bQ;
@@ -915,9 +917,10 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
     // aVS = |Uh[bn]| / sqrt(roS)
     {
       const double sg = slope_abs(Uh[bn]);
-      bUh[bn] += sg * (baVL / sqrt(roL) + baVR / sqrt(roR));
-      broL -= baVL * aVL / (2 * roL);
-      broR -= baVR * aVR / (2 * roR);
+      const double irL = 1.0 / roL, irR = 1.0 / roR;
+      bUh[bn] += sg * (baVL * sroL * irL + baVR * sroR * irR);  // 1/sqrt(ro) = sqrt(ro)/ro
+      broL -= 0.5 * baVL * aVL * irL;
+      broR -= 0.5 * baVR * aVR * irR;
     }
     // roS = S[RO] (aS - S[vn]) / (aS - aM)
     for (int side = 1; side >= 0; side--) {
@@ -928,9 +931,8 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
       const double roS = Ls ? roL : roR;
       const double b = Ls ? broL : broR;
       double &baS = Ls ? baL : baR;
-      const double den = aS - aM;
-      const double bnum = b / den;
-      const double bden = -b * roS / den;
+      const double bnum = b / (aS - aM);
+      const double bden = -bnum * roS;
       bS[RO] += (aS - S[vn]) * bnum;
       baS += S[RO] * bnum + bden;
       bS[vn] -= S[RO] * bnum;
@@ -947,8 +949,11 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
       baM += bc2;
     }
     // aM = Uh[mn] / Uh[RO]
-    bUh[mn] += baM / Uh[RO];
-    bUh[RO] -= baM * aM / Uh[RO];
+    {
+      const double iro = 1.0 / Uh[RO];
+      bUh[mn] += baM * iro;
+      bUh[RO] -= baM * aM * iro;
+    }
     // Uh[m] = iw (aR XR[m] - aL XL[m] - FR[m] + FL[m])
     for (int m = 0; m < 8; m++) {
       if (m == 3) continue;

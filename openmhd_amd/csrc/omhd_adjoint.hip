@@ -12,16 +12,16 @@
 //   lam*  = c1 lam** + (dD/dW at U*)^T (a2 lam**)
 //   lamg  = c0 lam** + lam* + (dD/dW at Ug)^T (a1 lam*)
 //   lam   = G lamg
-// Each (dD/dW)^T mu is 1 + 2 x (number of axes) launches over the grid:
+// Each (dD/dW)^T mu is 2 + (number of axes) launches over the grid:
 //   k_adj_prim     primitives of W (u2v)
 //   k_adj_face<x>  per x-face: recompute the two limited face states and the
 //                  HLLD flux, run its reverse on bF = mu(c+1) - mu(c), store the
 //                  adjoints of the face states by owning cell (AL at c, AR at c+1)
-//   k_adj_gather<x> per cell: the limiter reverse gathered from the three cells
-//                  whose slopes read it, into bw
-//   k_adj_face<y>, k_adj_gather<y> (and z): the same along y (z); the last
-//                  gather adds the resistive B adjoints, runs the u2v reverse
-//                  and the RK/GLM combine above, written in place.
+//   k_adj_face<y> (and z): the same along y (z), into their own AL/AR
+//   k_adj_gather   per cell: the limiter reverse gathered, per axis, from the
+//                  three cells whose slopes read it; the resistive B
+//                  adjoints; the u2v reverse and the RK/GLM combine above,
+//                  written in place.
 // Walls: a wall face's states are mirrors of the inside face state, so its
 // reverse folds both state adjoints onto the inside one; the slopes of the
 // two edge cells read the cell itself through its mirror ghost.
@@ -296,58 +296,54 @@ struct Combine {
   double *out;
 };
 
-// MODE 0: bw = contribution; 1: bw += contribution; 2: finish (resistive B
-// adjoints, u2v reverse and the RK/GLM combine into c.out)
-template <int DIR, int LIM, int MODE>
+// One thread per cell, all axes: the limiter reverse of the three cells per
+// axis whose slopes read this cell (coefficient index 1 - s for the slope at
+// q = c + s; at a wall, the edge cells q = 0 and q = n - 1 also read it
+// through their mirror ghost), then the resistive B adjoints, the u2v
+// reverse and the RK/GLM combine into c.out.  AL/AR hold one array per axis.
+template <int LIM, int ND>
 __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *__restrict__ w,
                                                     const double *__restrict__ AL, const double *__restrict__ AR,
-                                                    double *__restrict__ bw, const double *__restrict__ R,
-                                                    const Combine c) {
+                                                    long long astride, const double *__restrict__ R, const Combine c) {
   int i, j, k;
   if (!cell_of(g, i, j, k)) return;
-  const int cc = DIR == 0 ? i : (DIR == 1 ? j : k);
-  const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
-  const bool wall = g.bc[DIR] != OMHD_BC_PERIODIC;
-  Cell o[5];
-#pragma unroll
-  for (int s = 0; s < 5; s++) o[s] = nbr<DIR>(g, i, j, k, s - 2);
-  const long long o0 = o[2].o;
+  const long long o0 = (long long)k * g.zstride + (long long)j * g.pitch + i;
   double b[NV], wc[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    double v[5];
+    wc[m] = w[m * g.plane + o0];
+    b[m] = 0.0;
+  }
 #pragma unroll
-    for (int s = 0; s < 5; s++) v[s] = ld(w, g, o[s], m);
-    wc[m] = v[2];
-    double acc = 0.0;
-    // the slopes at cells q = c + s (s = -1, 0, 1) read this cell as their
-    // (wp, w0, wm) neighbour: coefficient index 1 - s; at a wall, q = 0 and
-    // q = n - 1 also read it through their mirror ghost
+  for (int d = 0; d < ND; d++) {
+    const int cc = d == 0 ? i : (d == 1 ? j : k);
+    const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+    const bool wall = g.bc[d] != OMHD_BC_PERIODIC;
+    Cell o[5];
 #pragma unroll
-    for (int s = -1; s <= 1; s++) {
-      const int q = cc + s;
-      if (wall && (q < 0 || q > n - 1)) continue;
-      double cR[3], cL[3];
-      adj::lim_coef(LIM, v[s + 1], v[s + 2], v[s + 3], cR, cL);
-      const double l = AL[m * g.plane + o[s + 2].o], r = AR[m * g.plane + o[s + 2].o];
-      if (s == 0) acc += l + r;  // gL = w + gradL, gR = w - gradR
-      acc += cL[1 - s] * l - cR[1 - s] * r;
-      if (wall && s == 0 && q == 0) acc += flip(cL[0] * l - cR[0] * r, g.slo[DIR], m);
-      if (wall && s == 0 && q == n - 1) acc += flip(cL[2] * l - cR[2] * r, g.shi[DIR], m);
+    for (int s = 0; s < 5; s++)
+      o[s] = cell_at(g, i + (d == 0 ? s - 2 : 0), j + (d == 1 ? s - 2 : 0), k + (d == 2 ? s - 2 : 0));
+    const double *al = AL + d * astride, *ar = AR + d * astride;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      double v[5];
+#pragma unroll
+      for (int s = 0; s < 5; s++) v[s] = ld(w, g, o[s], m);
+      double acc = 0.0;
+#pragma unroll
+      for (int s = -1; s <= 1; s++) {
+        const int q = cc + s;
+        if (wall && (q < 0 || q > n - 1)) continue;
+        double cR[3], cL[3];
+        adj::lim_coef(LIM, v[s + 1], v[s + 2], v[s + 3], cR, cL);
+        const double l = al[m * g.plane + o[s + 2].o], r = ar[m * g.plane + o[s + 2].o];
+        if (s == 0) acc += l + r;  // gL = w + gradL, gR = w - gradR
+        acc += cL[1 - s] * l - cR[1 - s] * r;
+        if (wall && s == 0 && q == 0) acc += flip(cL[0] * l - cR[0] * r, g.slo[d], m);
+        if (wall && s == 0 && q == n - 1) acc += flip(cL[2] * l - cR[2] * r, g.shi[d], m);
+      }
+      b[m] += acc;
     }
-    b[m] = acc;
-  }
-  if (MODE == 0) {
-#pragma unroll
-    for (int m = 0; m < NV; m++) bw[m * g.plane + o0] = b[m];
-    return;
-  }
-#pragma unroll
-  for (int m = 0; m < NV; m++) b[m] += bw[m * g.plane + o0];
-  if (MODE == 1) {
-#pragma unroll
-    for (int m = 0; m < NV; m++) bw[m * g.plane + o0] = b[m];
-    return;
   }
   if (R) {
 #pragma unroll
@@ -379,7 +375,7 @@ __global__ void k_lam_copy(double *__restrict__ dst, const double *__restrict__ 
 
 typedef void (*face_fn)(const Geo, const double *, const double *, double, double, double, double *, double *);
 typedef void (*res_fn)(const Geo, const double *, const double *, double, double, double *, double *);
-typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, double *, const double *,
+typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, long long, const double *,
                           const Combine);
 
 template <int DIR, int LO>
@@ -401,22 +397,15 @@ face_fn face_kernel(int dir, int lo, int lim) {
 
 res_fn res_kernel(int dir) { return dir == 0 ? k_adj_res<0> : (dir == 1 ? k_adj_res<1> : k_adj_res<2>); }
 
-template <int DIR, int MODE>
+template <int ND>
 gather_fn gather_kernel(int lim) {
   switch (lim) {
-    case 0: return k_adj_gather<DIR, 0, MODE>;
-    case 1: return k_adj_gather<DIR, 1, MODE>;
-    case 2: return k_adj_gather<DIR, 2, MODE>;
-    case 3: return k_adj_gather<DIR, 3, MODE>;
-    default: return k_adj_gather<DIR, 4, MODE>;
+    case 0: return k_adj_gather<0, ND>;
+    case 1: return k_adj_gather<1, ND>;
+    case 2: return k_adj_gather<2, ND>;
+    case 3: return k_adj_gather<3, ND>;
+    default: return k_adj_gather<4, ND>;
   }
-}
-
-// x writes bw, y adds (3D) or finishes (2D), z finishes
-gather_fn gather_kernel(int dir, int nd, int lim) {
-  if (dir == 0) return gather_kernel<0, 0>(lim);
-  if (dir == 1) return nd == 3 ? gather_kernel<1, 1>(lim) : gather_kernel<1, 2>(lim);
-  return gather_kernel<2, 2>(lim);
 }
 
 }  // namespace
@@ -431,17 +420,16 @@ struct omhd_adjoint {
   size_t state = 0;            // doubles per state buffer (9 planes)
   double *ckpt = nullptr;      // nseg states: U at steps 0, every, 2 every, ...
   double *tape = nullptr;      // 2 * every states: (U^n, U*^n) of one segment
-  double *work = nullptr;      // w, AL, AR, bw, lam, lam*
+  double *work = nullptr;      // w, AL[3 axes], AR[3 axes], lam, lam*
   double *R = nullptr;         // resistive: B adjoints (3 planes)
   double *geta = nullptr;      // resistive: adjoint of each face's eta, 3 (nx+2)(ny+2)
   std::vector<double> geta_host;
   bool have_forward = false, have_backward = false;
   double *w() { return work; }
-  double *AL() { return work + state; }
-  double *AR() { return work + 2 * state; }
-  double *bw() { return work + 3 * state; }
-  double *lam() { return work + 4 * state; }
-  double *lams() { return work + 5 * state; }
+  double *AL(int d = 0) { return work + (1 + d) * state; }
+  double *AR(int d = 0) { return work + (4 + d) * state; }
+  double *lam() { return work + 7 * state; }
+  double *lams() { return work + 8 * state; }
   size_t neta() const { return (size_t)3 * (h->cfg.nx + 2) * (h->cfg.ny + 2); }
 };
 
@@ -478,7 +466,7 @@ extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every
   a->ch.assign(nsteps, 0.0);
   hipError_t e = hipMalloc((void **)&a->ckpt, a->state * a->nseg * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&a->tape, a->state * 2 * every * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void **)&a->work, a->state * 6 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&a->work, a->state * 9 * sizeof(double));
   if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->R, (size_t)h->plane * 3 * sizeof(double));
   if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->geta, a->neta() * sizeof(double));
   if (e != hipSuccess) {
@@ -538,12 +526,12 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
   OMHD_HIP(hipGetLastError());
   for (int dir = 0; dir < g.nd; dir++) {
     hipLaunchKernelGGL(face_kernel(dir, 0, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch,
-                       a->AL(), a->AR());
+                       a->AL(dir), a->AR(dir));
     OMHD_HIP(hipGetLastError());
     if (g.bc[dir] != OMHD_BC_PERIODIC) {
       const long long nt = (long long)n / (dir == 0 ? g.nx : (dir == 1 ? g.ny : g.nz));
       hipLaunchKernelGGL(face_kernel(dir, 1, lim), dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, h->stream, g,
-                         a->w(), M, mscale, mpsi, ch, a->AL(), a->AR());
+                         a->w(), M, mscale, mpsi, ch, a->AL(dir), a->AR(dir));
       OMHD_HIP(hipGetLastError());
     }
     if (res) {
@@ -551,11 +539,10 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
                          a->geta);
       OMHD_HIP(hipGetLastError());
     }
-    const bool last = dir == g.nd - 1;
-    hipLaunchKernelGGL(gather_kernel(dir, g.nd, lim), dim3(g256), dim3(256), 0, h->stream, g, a->w(), a->AL(), a->AR(),
-                       a->bw(), res ? a->R : nullptr, last ? fin : Combine{});
-    OMHD_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(g.nd == 3 ? gather_kernel<3>(lim) : gather_kernel<2>(lim), dim3(g256), dim3(256), 0, h->stream, g,
+                     a->w(), a->AL(), a->AR(), (long long)a->state, res ? a->R : nullptr, fin);
+  OMHD_HIP(hipGetLastError());
   return OMHD_OK;
 }
 
