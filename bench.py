@@ -85,7 +85,7 @@ def cpu_baseline(n: int, sample_steps: int):
 
 
 def bench_ensemble(a):
-    """Config C3: 256 independent KH runs, sides log-uniform in [64, 1024],
+    """256 independent KH runs, sides log-uniform in [64, 1024],
     300 steps, each with its own dt.  Batched (one launch per RK stage over all
     members, autotuned tiles; host dt and device dt + hipGraph) against the same
     members run one after another with the single-grid step."""
@@ -132,11 +132,11 @@ def bench_ensemble(a):
         out[mode] = {"seconds": el, "zone_updates_per_s": cells * steps / el, "speedup_vs_sequential": seq / el,
                      "tile_rows": rows, "grid": grid}
     out["sequential_zone_updates_per_s"] = cells * steps / seq
-    print(json.dumps({"workload": "C3 KH ensemble", **out}), flush=True)
+    print(json.dumps({"workload": "KH ensemble, 256 members, sides 64-1024, 300 steps", **out}), flush=True)
 
 
 def bench_3d(a):
-    """Config C5: 3D resistive Harris sheet 512x256x256 (Bx = tanh(y), rho =
+    """3D resistive Harris sheet 512x256x256 (Bx = tanh(y), rho =
     0.2 + cosh^-2(y)/0.2, eta = 1e-3 + 1e-2 exp(-(x^2+y^2)/4), 1e-3 velocity
     noise), HLLD + resistive flux + GLM + TVD-RK2, minmod, x/z periodic, y
     walls: forward zone-updates/s over `steps` steps; then (unless --no-grad)
@@ -166,7 +166,7 @@ def bench_3d(a):
     el = time.perf_counter() - t0
     ms, n = s.timing()
     cells = nx * ny * nz
-    res = {"workload": f"C5 3D {'ideal' if a.no_res else 'resistive'} Harris {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
+    res = {"workload": f"3D {'ideal' if a.no_res else 'resistive'} Harris sheet {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
            "zone_updates_per_s": cells * a.steps / el,
            "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
            "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}
@@ -235,7 +235,7 @@ def bench_3d(a):
 
 
 def bench_adjoint(a):
-    """Config C4: Orszag-Tang 512^2 + 16 seeded Fourier velocity modes
+    """Orszag-Tang 512^2 + 16 seeded Fourier velocity modes
     (|k| <= 8, amplitude 1e-2), 200 MC/HLLD/GLM/TVD-RK2 steps, checkpoints
     every 20; loss = ||By(T) - By_target(T)||^2 with the target run from a
     second seeded perturbation; gradient w.r.t. the 16 mode amplitudes via the
@@ -320,7 +320,7 @@ def bench_adjoint(a):
         torch.cuda.synchronize()
         t["backward"] = time.perf_counter() - t0
     adj.close()
-    res = {"workload": f"C4 adjoint OT {n}^2 + 16 modes, {nsteps} steps, checkpoints every {every}",
+    res = {"workload": f"adjoint: Orszag-Tang {n}^2 + 16 modes, {nsteps} steps, checkpoints every {every}",
            "loss": float(L), "grad_norm": float(np.linalg.norm(grad)), "fd_eps": list(eps_list),
            "fd_rel_err_median_per_eps": [float(x) for x in np.median(rel, axis=0)],
            "fd_rel_err_best_max": float(best.max()), "fd_rel_err_best_median": float(np.median(best)),
@@ -340,7 +340,8 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d", "adjoint"),
-                    help="kh: the headline line; ensemble / adjoint / 3d: configurations C3 / C4 / C5 (own JSON lines)")
+                    help="kh: KH 4096^2 (the headline line); ensemble: 256 KH members; adjoint: OT 512^2 gradient; "
+                         "3d: Harris sheet 512x256x256 forward + resistivity gradient (own JSON lines)")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--nx3", type=int, default=512)
     ap.add_argument("--ny3", type=int, default=256)
@@ -352,8 +353,8 @@ def main():
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        sys.exit("bench.py: this build runs on one GPU only (the grid is one domain on one MI355X); "
-                 "multi-GPU sharding is a design note, not built -- run with --gpus 1")
+        sys.exit("bench.py runs on one GPU: the grid is one domain on one MI355X and sharding it "
+                 "across GPUs is not built -- run with --gpus 1")
 
     if a.workload != "kh":
         sys.path.insert(0, str(REPO))
@@ -426,7 +427,7 @@ def main():
                                f"TVD-RK2, CFL 0.4", "nx": n, "ny": n, "limiter": "minmod", "flux": "HLLD",
                    "time": "TVD-RK2", "parallelism": "single GPU, one domain"},
         "hbm_frac_step": zups * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
-        "roofline": {"bound": "hbm", "kernel": f"k_sweep2d stage {dom + 1}", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"k_march stage {dom + 1} (k_march<LIM,{dom + 1},TT,0,0>)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": st[dom], "algorithmic_bytes_per_launch": BYTES_STAGE[dom + 1] * cells},
         "cpu_baseline": cpu,

@@ -28,6 +28,7 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -435,9 +436,12 @@ __global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const do
 template <int ND>
 __global__ void k_cfl(const double *__restrict__ s, long long plane, long long zstride, int pitch, int nx, int ny,
                       int nz, unsigned long long *vmax_bits, int *err) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride: a bounded grid keeps the final atomics to one per wavefront
+  // of the grid, not one per 64 cells (they all hit one address)
+  const long long n = (long long)nx * ny * nz;
   double vt = 0.0;
-  if (gid < (long long)nx * ny * nz) {
+  for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
+       gid += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(gid % nx);
     const long long r = gid / nx;
     const int j = (int)(r % ny), k = (int)(r / ny);
@@ -449,11 +453,9 @@ __global__ void k_cfl(const double *__restrict__ s, long long plane, long long z
     const int kk = ND == 3 ? k + 2 : 0;
     if (!(w[3] > 0.0) || !(u[4] > 0.0))
       record_error(err, (w[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, i + 2, j + 2, kk);
-    vt = fastest_speed<ND>(u, w);
-    if (!(vt >= 0.0)) {
-      record_error(err, OMHD_ERR_DT, i + 2, j + 2, kk);
-      vt = 0.0;
-    }
+    const double c = fastest_speed<ND>(u, w);
+    if (!(c >= 0.0)) record_error(err, OMHD_ERR_DT, i + 2, j + 2, kk);
+    else vt = dmax(vt, c);
   }
   vt = wave_max(vt);
   if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax_bits, vt);
@@ -685,7 +687,7 @@ static int cfl_pass(omhd_handle *h) {
   const omhd_config &c = h->cfg;
   k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
   const size_t n = ncells(h);
-  const unsigned grid = (unsigned)((n + 255) / 256);
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
   if (h->nd == 3)
     k_cfl<3><<<grid, 256, 0, h->stream>>>(h->U, h->plane, h->zstride, h->pitch, c.nx, c.ny, c.nz, h->red,
                                           (int *)(h->red + 1));

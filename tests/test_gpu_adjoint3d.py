@@ -1,5 +1,5 @@
 """The GPU adjoint with wall boundaries, in 3D, and with the resistive flux
-(config C5's physics on small grids), against the CPU reverse of the same
+(the 3D resistive reconnection physics on small grids), against the CPU reverse of the same
 discrete step (tests/native/adjcheck.cpp, general form: its forward is the
 oracle's bits, its reverse the exact transpose of dual-number tangents), and
 the resistivity-parameter gradient against the exact tangent along d eta /

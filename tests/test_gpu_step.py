@@ -79,7 +79,7 @@ def test_golden_kh_walls():
 
 
 def test_orszag_tang_1000_steps():
-    """Config C1 (OT 200^2, MC, 1000 steps): dt sequence and final state
+    """OT 200^2, MC, 1000 steps: dt sequence and final state
     bitwise equal to the reference CPU build."""
     d = load("ot200_mc_1000")
     U, dx = orszag_tang(200)
@@ -98,7 +98,7 @@ def test_orszag_tang_1000_steps():
 
 
 def test_kh_4096_two_steps_vs_oracle(oracle):
-    """Config C2's grid (4096^2 KH, minmod, walls): two full steps on the GPU
+    """The KH grid of the benchmark (4096^2, minmod, walls): two full steps on the GPU
     equal the oracle's bitwise at full size."""
     from openmhd_amd.problems import kelvin_helmholtz
 

@@ -63,7 +63,7 @@ def test_golden_3d_run():
 
 @pytest.mark.parametrize("bc", (("periodic", "wall", "periodic"), ("wall", "wall", "periodic")))
 def test_resistive_harris(oracle, bc):
-    """Config C5's physics (Harris sheet, localized eta, HLLD + resistive flux +
+    """The 3D reconnection physics (Harris sheet, localized eta, HLLD + resistive flux +
     GLM + TVD-RK2, resistive dt limit) on a small grid."""
     from openmhd_amd import problems
 

@@ -118,7 +118,7 @@ def test_kh_walls_run(oracle):
 
 
 def test_orszag_tang_1000_steps(oracle):
-    """Config C1: OT 200^2, MC, HLLD, GLM, TVD-RK2, CFL 0.4, 1000 steps: the
+    """OT 200^2, MC, HLLD, GLM, TVD-RK2, CFL 0.4, 1000 steps: the
     dt sequence and the final state are bitwise those of the reference."""
     d = load("ot200_mc_1000")
     U, dx = orszag_tang(200)
