@@ -39,11 +39,12 @@ def make_config(nx, ny, dx, nz=0, cfl=0.4, lm_type=2, flux_type=3, time_type=0, 
     cfg.dx, cfg.cfl = dx, cfl
     cfg.lm_type, cfg.flux_type, cfg.time_type = lm_type, flux_type, time_type
     per_axis = len(wall_sign) == 3 and hasattr(wall_sign[0], "__len__")
+    per_face = len(wall_sign) == 6 and hasattr(wall_sign[0], "__len__")
     for a in range(3):
         kind = bc[a] if a < len(bc) else "periodic"
-        sg = wall_sign[a] if per_axis else wall_sign
         for s in range(2):
             f = 2 * a + s
+            sg = wall_sign[f] if per_face else (wall_sign[a] if per_axis else wall_sign)
             cfg.bc_type[f] = OMHD_BC_MIRROR if kind == "wall" else OMHD_BC_PERIODIC
             for m in range(9):
                 cfg.bc_sign[9 * f + m] = sg[m] if kind == "wall" else 1.0
@@ -61,7 +62,8 @@ class Solver:
 
     bc: per axis ('periodic' | 'wall'); wall_sign: 9 signs (+1/-1) applied to
     the mirrored ghost (default: the KH walls of 2D_KH/bc.f90), or a list of
-    one 9-vector per axis.  resistivity: (eta0, eta1, w) for the 3D resistive
+    one 9-vector per axis, or of one per face (x-, x+, y-, y+, z-, z+; e.g.
+    the 2D_reconnection/bc.f90 tables).  resistivity: (eta0, eta1, w) for the 3D resistive
     step with eta = eta0 + eta1 exp(-(x^2+y^2)/w^2) at the face centres;
     origin: (x0, y0), the lower corner of the domain.
     """

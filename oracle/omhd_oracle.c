@@ -28,9 +28,13 @@
  *                     the flang build; KH walls 100 steps; 3D periodic 4 steps
  *   or_eta_faces,     closed form of the resistivity profile; the resistive dt
  *   or_set_dt_resistive  limit 0.5 cfl dx^2 / eta_max (flux_resistive.cuf:2-9)
- *   or_flux_resistive parity unpinned against a reference CPU build (the
- *                     reference has only the CUDA Fortran 3D form); pinned by
- *                     closed forms (uniform B -> no change, linear B -> const J)
+ *   or_flux_resistive the reference's 3D form is CUDA Fortran only; pinned by
+ *                     the flang build of its 2D CPU form on z-invariant fields
+ *                     (2D_reconnection/flux_resistive.f90, which folds eta/dx
+ *                     into the currents: equal to rounding, 2e-15 relative) and
+ *                     by closed forms (uniform B -> no flux, linear B -> const J)
+ *   or_bc_u/or_bc_face per-face sign tables: flang vectors of 2D_KH/bc.f90 and
+ *                     2D_reconnection/bc.f90
  */
 #include <math.h>
 #include <stdint.h>

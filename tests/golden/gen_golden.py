@@ -6,7 +6,7 @@ subroutines on crafted inputs and stores inputs + outputs as .npz (plain
 arrays, no pickles).  Branch labels for the HLLD vectors come from the oracle
 and are used only to choose inputs that cover every branch.
 
-    python tests/golden/gen_golden.py        (needs oracle/_ref, built here)
+    python tests/golden/gen_golden.py [3d | recon]   (needs oracle/_ref, built here)
 """
 from __future__ import annotations
 
@@ -239,8 +239,49 @@ def only_3d():
     save("run3d", U0=U0, U=U, dt=dts, ch=chs)
 
 
+def recon():
+    """2D_reconnection: per-face boundary tables (bc_for_U/F/G) and the 2D CPU
+    form of the resistive flux (its own eta(x, y) profile)."""
+    from omhd_testlib import RefRec  # noqa: E402
+
+    rr = RefRec()
+    rng = np.random.default_rng(44)
+    U = farr(9, 7, 9)
+    U[...] = rng.normal(size=U.shape)
+    Uo = U.copy(order="F")
+    rr.bc_for_U(Uo)
+    VL, VR = farr(9, 7, 9), farr(9, 7, 9)
+    VL[...] = rng.normal(size=VL.shape)
+    VR[...] = rng.normal(size=VR.shape)
+    FL, FR = VL.copy(order="F"), VR.copy(order="F")
+    rr.bc_for_F(FL, FR)
+    GL, GR = VL.copy(order="F"), VR.copy(order="F")
+    rr.bc_for_G(GL, GR)
+    # resistive flux on a smooth field near the resistivity peak (x = y = 0)
+    ix, jx, dx = 12, 10, 0.3
+    x = -0.5 * (ix - 2) * dx - 0.5 * dx + dx * np.arange(ix)
+    y = -0.5 * (jx - 2) * dx - 0.5 * dx + dx * np.arange(jx)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    Ur = farr(ix, jx, 9)
+    Ur[...] = 0.1 * rng.normal(size=Ur.shape)
+    Ur[..., 5] = np.tanh(Y) + 0.2 * np.sin(X)
+    Ur[..., 6] = 0.3 * np.sin(0.7 * X + 0.4 * Y)
+    Ur[..., 7] = 0.5 * np.cos(0.5 * X - 0.9 * Y)
+    out = {}
+    for d in (1, 2):
+        F = farr(ix, jx, 9)
+        F[...] = rng.normal(size=F.shape)
+        out[f"F0_{d}"] = F.copy(order="F")
+        rr.flux_resistive(F, Ur, x, y, dx, d)
+        out[f"F_{d}"] = F
+    save("recon", bcU_in=U, bcU_out=Uo, V_L=VL, V_R=VR, bcF_L=FL, bcF_R=FR, bcG_L=GL, bcG_R=GR, res_U=Ur, res_x=x,
+         res_y=y, res_dx=dx, **out)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "3d":
         only_3d()
+    elif len(sys.argv) > 1 and sys.argv[1] == "recon":
+        recon()
     else:
         main()

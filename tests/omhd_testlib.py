@@ -53,15 +53,20 @@ class OrProblem(C.Structure):
 def make_problem(ix, jx, kx=1, dx=1.0, cfl=0.4, lm_type=2, flux_type=3, time_type=0, bc=("periodic",) * 3,
                  signs=None):
     """bc: per direction 'periodic' or 'wall' (mirror with a sign table).
-    signs: per direction a 9-vector of +-1 for the mirror (default KH walls)."""
+    signs: per direction a 9-vector of +-1 for the mirror (default KH walls),
+    or 6 9-vectors, one per face (x-, x+, y-, y+, z-, z+)."""
     p = OrProblem()
     p.ix, p.jx, p.kx = ix, jx, kx
     p.dx, p.cfl = dx, cfl
     p.lm_type, p.flux_type, p.time_type = lm_type, flux_type, time_type
+    per_face = signs is not None and len(signs) == 6
     for d in range(3):
         wall = bc[d] == "wall"
-        sg = KH_SIGN if signs is None or signs[d] is None else np.asarray(signs[d], dtype=np.float64)
         for s in range(2):
+            if per_face:
+                sg = np.asarray(signs[2 * d + s], dtype=np.float64)
+            else:
+                sg = KH_SIGN if signs is None or signs[d] is None else np.asarray(signs[d], dtype=np.float64)
             p.bctype[2 * d + s] = 1 if wall else 0
             for m in range(9):
                 p.u_sgn[(2 * d + s) * 9 + m] = sg[m] if wall else 1.0
@@ -227,6 +232,42 @@ class Ref:
 
     def bc_for_G(self, VL, VR):
         self.KH.bc_for_g_(ptr(VL), ptr(VR), *self._ext(VL))
+
+
+class RefRec:
+    """2D_reconnection's boundary tables (bc.f90) and the 2D CPU form of the
+    resistive flux (flux_resistive.f90), compiled unmodified by oracle/Makefile."""
+
+    def __init__(self):
+        so = REF_DIR / "librefrec.so"
+        if not so.exists():
+            raise FileNotFoundError(so)
+        self.L = C.CDLL(str(so))
+
+    def bc_for_U(self, U):
+        self.L.bc_for_u_(ptr(U), _i(U.shape[0]), _i(U.shape[1]))
+
+    def bc_for_F(self, VL, VR):
+        self.L.bc_for_f_(ptr(VL), ptr(VR), _i(VL.shape[0]), _i(VL.shape[1]))
+
+    def bc_for_G(self, VL, VR):
+        self.L.bc_for_g_(ptr(VL), ptr(VR), _i(VL.shape[0]), _i(VL.shape[1]))
+
+    def flux_resistive(self, F, U, x, y, dx, dir):
+        self.L.flux_resistive_(ptr(F), ptr(U), ptr(np.ascontiguousarray(x)), ptr(np.ascontiguousarray(y)), _d(dx),
+                               _i(U.shape[0]), _i(U.shape[1]), _i(dir))
+
+
+# 2D_reconnection/bc.f90: slots negated in the mirror ghosts, per face
+# (x-, x+, y-, y+): x conducting (mx, mz, By, psi), south symmetry plane
+# (my, mz, Bx, psi), north wall (my, By)
+def _neg(*slots):
+    s = np.ones(9)
+    s[list(slots)] = -1.0
+    return s
+
+
+RECON_SIGN = [_neg(0, 2, 6, 8), _neg(0, 2, 6, 8), _neg(1, 2, 5, 8), _neg(1, 6), np.ones(9), np.ones(9)]
 
 
 def ref_available(dim=2) -> bool:

@@ -26,14 +26,15 @@ def soa3(U, nd):
     return np.ascontiguousarray(U[1:-1, 1:-1, 1:-1, :].transpose(3, 2, 1, 0))
 
 
-def setup(oracle, nx, ny, nz, bc, lm, tt, res, seed, dx=0.1):
+def setup(oracle, nx, ny, nz, bc, lm, tt, res, seed, dx=0.1, signs=None):
     """Solver + oracle problem + CPU harness arguments for one case."""
     from openmhd_amd import Solver, snapshot
 
     rng = np.random.default_rng(seed)
     nd = 3 if nz else 2
     U = smooth_state(rng, nx + 2, ny + 2, (nz + 2) if nz else 1)
-    p = make_problem(nx + 2, ny + 2, (nz + 2) if nz else 1, dx=dx, cfl=0.3, lm_type=lm, time_type=tt, bc=bc)
+    p = make_problem(nx + 2, ny + 2, (nz + 2) if nz else 1, dx=dx, cfl=0.3, lm_type=lm, time_type=tt, bc=bc,
+                     signs=signs)
     origin = (-0.5 * nx * dx, -0.5 * ny * dx)
     resistivity = (1e-3, 2e-2, 0.5) if res else None
     eta = None
@@ -45,27 +46,33 @@ def setup(oracle, nx, ny, nz, bc, lm, tt, res, seed, dx=0.1):
         p.eta = eta.ctypes.data_as(p._fields_[-2][1])
         p.eta_max = emax
     oracle.bc_u(U, p)
-    s = Solver(nx, ny, dx, nz=nz, cfl=0.3, lm_type=lm, time_type=tt, bc=bc, resistivity=resistivity, origin=origin)
+    s = Solver(nx, ny, dx, nz=nz, cfl=0.3, lm_type=lm, time_type=tt, bc=bc, resistivity=resistivity, origin=origin,
+               **({"wall_sign": signs} if signs is not None else {}))
     s.set_state(soa3(U, nd))
     bca = (C.c_int * 3)(*[1 if b == "wall" else 0 for b in bc])
     sg = np.ascontiguousarray(np.array(p.u_sgn[:], dtype=np.float64))
     return s, U, p, soa3(U, nd), eta, bca, sg, resistivity, origin
 
 
+from omhd_testlib import RECON_SIGN  # noqa: E402
+
 CASES = [
-    (12, 10, 0, ("periodic", "wall", "periodic"), 1, 0, 0, 4, 2),
-    (12, 10, 0, ("wall", "wall", "periodic"), 2, 1, 0, 3, 3),
-    (8, 6, 5, ("periodic",) * 3, 2, 0, 0, 3, 2),
-    (8, 6, 5, ("periodic", "wall", "periodic"), 1, 0, 1, 3, 2),
-    (8, 6, 6, ("wall", "wall", "wall"), 2, 1, 1, 2, 1),
+    (12, 10, 0, ("wall", "wall", "periodic"), 2, 0, 0, 3, 2, RECON_SIGN),
+    (8, 6, 5, ("wall", "wall", "periodic"), 1, 0, 1, 3, 2, RECON_SIGN),
+    (12, 10, 0, ("periodic", "wall", "periodic"), 1, 0, 0, 4, 2, None),
+    (12, 10, 0, ("wall", "wall", "periodic"), 2, 1, 0, 3, 3, None),
+    (8, 6, 5, ("periodic",) * 3, 2, 0, 0, 3, 2, None),
+    (8, 6, 5, ("periodic", "wall", "periodic"), 1, 0, 1, 3, 2, None),
+    (8, 6, 6, ("wall", "wall", "wall"), 2, 1, 1, 2, 1, None),
 ]
 
 
-@pytest.mark.parametrize("nx,ny,nz,bc,lm,tt,res,nsteps,every", CASES)
-def test_adjoint_matches_cpu_reverse(ac, oracle, nx, ny, nz, bc, lm, tt, res, nsteps, every):
+@pytest.mark.parametrize("nx,ny,nz,bc,lm,tt,res,nsteps,every,signs", CASES)
+def test_adjoint_matches_cpu_reverse(ac, oracle, nx, ny, nz, bc, lm, tt, res, nsteps, every, signs):
     from openmhd_amd import Adjoint
 
-    s, U, p, W0, eta, bca, sg, resistivity, origin = setup(oracle, nx, ny, nz, bc, lm, tt, res, nx + 3 * res)
+    s, U, p, W0, eta, bca, sg, resistivity, origin = setup(oracle, nx, ny, nz, bc, lm, tt, res, nx + 3 * res,
+                                                           signs=signs)
     adj = Adjoint(s, nsteps, every)
     dts, chs = adj.forward()
     Uo = U.copy(order="F")

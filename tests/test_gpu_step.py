@@ -118,3 +118,29 @@ def test_negative_pressure_is_reported():
     with pytest.raises(OMHDError) as e:
         s.upload(U)
     assert e.value.code == 1 and e.value.cell[:2] == (11, 13)
+
+
+@pytest.mark.parametrize("rows", (5, 24))
+def test_reconnection_boundary_tables(oracle, rows):
+    """2D_reconnection/bc.f90's per-face tables: x conducting walls, a south
+    symmetry plane and a north wall that negate different slots (the oracle's
+    tables are pinned to that file's bc_for_U/F/G): dt sequence and state
+    bitwise."""
+    from omhd_testlib import RECON_SIGN
+
+    from openmhd_amd import Solver
+
+    rng = np.random.default_rng(77)
+    U = smooth_state(rng, 70, 38)
+    nx, ny = 68, 36
+    p = make_problem(nx + 2, ny + 2, dx=0.05, cfl=0.4, lm_type=2, bc=("wall", "wall", "periodic"), signs=RECON_SIGN)
+    oracle.bc_u(U, p)
+    s = Solver(nx, ny, 0.05, cfl=0.4, lm_type=2, bc=("wall", "wall"), wall_sign=RECON_SIGN, rows_per_wave=rows)
+    s.upload(U)
+    dts, chs = s.step(6, trace=True)
+    Ug = s.download()
+    s.close()
+    Uo = U.copy(order="F")
+    rc, dto, cho = oracle.run(p, Uo, 6)
+    assert rc == 0
+    assert bits_equal(dts, dto) and bits_equal(chs, cho) and bits_equal(Ug, Uo)

@@ -194,3 +194,54 @@ def test_resistivity_profile_and_dt_closed_form(oracle):
     f.argtypes = [C.c_double] * 4
     assert f(1.0, 0.1, 0.4, 0.011) == 0.5 * 0.4 * (0.1 * 0.1) / 0.011
     assert f(1e-4, 0.1, 0.4, 0.011) == 1e-4
+
+
+def test_reconnection_boundary_tables(oracle):
+    """Per-face sign tables, against 2D_reconnection/bc.f90 (bitwise): x
+    conducting walls, a south symmetry plane and a north wall with different
+    negated slots on the two y faces; corners from x then y."""
+    from omhd_testlib import RECON_SIGN, ptr
+
+    d = load("recon")
+    U = np.asfortranarray(d["bcU_in"]).copy(order="F")
+    oracle.bc_u(U, make_problem(9, 7, bc=("wall", "wall", "periodic"), signs=RECON_SIGN))
+    assert bits_equal(U, d["bcU_out"])
+    for dr, kl, kr in ((1, "bcF_L", "bcF_R"), (2, "bcG_L", "bcG_R")):
+        VL = np.asfortranarray(d["V_L"]).copy(order="F")
+        VR = np.asfortranarray(d["V_R"]).copy(order="F")
+        lo = np.ascontiguousarray(RECON_SIGN[2 * dr - 2])
+        hi = np.ascontiguousarray(RECON_SIGN[2 * dr - 1])
+        oracle.L.or_bc_face(ptr(VL), ptr(VR), 9, 7, 1, dr, 1, ptr(lo), ptr(hi))
+        assert bits_equal(VL, d[kl]) and bits_equal(VR, d[kr]), dr
+
+
+def test_resistive_flux_against_the_2d_reference_form(oracle):
+    """The oracle's resistive flux (the 3D form of
+    3D_reconnection_gpu/flux_resistive.cuf) on a z-invariant field against the
+    reference's 2D CPU form (2D_reconnection/flux_resistive.f90) with that
+    file's eta(x, y) profile.  The 2D form folds eta/dx into the currents, so
+    the order of operations differs: agreement to rounding, not bitwise."""
+    import ctypes as C
+
+    from omhd_testlib import ptr
+
+    d = load("recon")
+    U2, x, y, dx = d["res_U"], d["res_x"], d["res_y"], float(d["res_dx"])
+    ix, jx = U2.shape[:2]
+    rm0, rm1 = 1000.0, 60.0
+    e0, e01 = 1.0 / rm0, (rm0 - rm1) / (rm0 * rm1)
+
+    def eta(xx, yy):
+        return e0 + e01 * np.cosh(np.minimum(np.sqrt(xx ** 2 + yy ** 2), 25.0)) ** -2.0
+
+    X, Y = np.meshgrid(x, y, indexing="xy")  # (jx, ix)
+    faces = np.ascontiguousarray(np.stack([eta(X + 0.5 * dx, Y), eta(X, Y + 0.5 * dx), eta(X, Y)]))
+    U3 = np.asfortranarray(np.repeat(U2[:, :, None, :], 3, axis=2))
+    oracle.L.or_flux_resistive.argtypes = [C.POINTER(C.c_double)] * 3 + [C.c_double] + [C.c_int] * 4
+    for dr in (1, 2):
+        F0 = d[f"F0_{dr}"]
+        F3 = np.asfortranarray(np.repeat(F0[:, :, None, :], 3, axis=2))
+        oracle.L.or_flux_resistive(ptr(F3), ptr(U3), ptr(np.ascontiguousarray(faces[dr - 1])), dx, ix, jx, 3, dr)
+        got, ref = F3[:, :, 1, :] - F0, d[f"F_{dr}"] - F0
+        assert np.abs(ref).max() > 1e-6  # the flux is not trivial here
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
