@@ -128,7 +128,10 @@ template <int RES>
 using LdsOf = typename std::conditional<RES != 0, SweepLdsRes, SweepLds>::type;
 
 // One tile: strip x chunk (x slice) of one grid.
-template <int LIM, int STAGE, int TT, int PASS, int RES>
+// PREF: stage 2 issues the U^n loads of the combined row early (where the
+// register budget allows it without spills: the resident kernels, not the
+// ensemble kernel, whose per-member parameters take more registers)
+template <int LIM, int STAGE, int TT, int PASS, int RES, bool PREF = true>
 __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RES> &lds) {
   constexpr bool XS = PASS != PASS_Z;         // x-sweep of the oldest row
   constexpr bool FINAL = PASS != PASS_XY;     // RK update (else write acc)
@@ -237,6 +240,16 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
       for (int m = 0; m < NV; m++) put66(lds.ring[sc][m], lane, W[m]);
     }
     __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
+    // stage 2: issue the loads of U^n for the row combined at the end of this
+    // iteration now, so their latency hides behind the two flux evaluations
+    double a0p[NV];
+    if constexpr (FINAL && STAGE == 2 && PREF) {
+      if (k >= 4 && writer) {
+        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
+#pragma unroll
+        for (int m = 0; m < NV; m++) a0p[m] = p.u0[m * plane + o];
+      }
+    }
 
     // x-sweep of row r = rn-2: tq = (F(i-1) - F(i)) + G(j-1)   (rk.f90:30-31 order)
     double tq[NV];
@@ -349,7 +362,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             } else {
 #pragma unroll
               for (int m = 0; m < NV; m++) {
-                double a0 = p.u0[m * plane + o];
+                double a0;
+                if constexpr (PREF) a0 = a0p[m];
+                else a0 = p.u0[m * plane + o];
                 if (m == 8) a0 = glm_f * a0;
                 if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + dtx * d[m]);  // rk_tvd22
                 else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
@@ -383,7 +398,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ LdsOf<RES> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
-  march_tile<LIM, STAGE, TT, PASS, RES>(p, t, lds);
+  march_tile<LIM, STAGE, TT, PASS, RES, (LIM != 0)>(p, t, lds);
 }
 
 // Ensemble: one persistent launch per RK stage for every member.  The tile
@@ -398,7 +413,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     p.us = STAGE == 1 ? p.U : p.U1;
     p.u0 = p.U;
     p.out = STAGE == 1 ? p.U1 : p.U;
-    march_tile<LIM, STAGE, TT, PASS_2D, 0>(p, tl.y, lds);
+    march_tile<LIM, STAGE, TT, PASS_2D, 0, false>(p, tl.y, lds);
     __syncthreads();
   }
 }
