@@ -242,12 +242,19 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
     // stage 2: issue the loads of U^n for the row combined at the end of this
     // iteration now, so their latency hides behind the two flux evaluations
-    double a0p[NV];
+    double a0p[NV], usp[NV];
     if constexpr (FINAL && STAGE == 2 && PREF) {
       if (k >= 4 && writer) {
         const long long o = fixoff + (long long)(rn - 2) * rstride + c;
 #pragma unroll
         for (int m = 0; m < NV; m++) a0p[m] = p.u0[m * plane + o];
+      }
+    }
+    if constexpr (FINAL && STAGE == 1 && PREF) {
+      if (k >= 4 && writer) {
+        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
+#pragma unroll
+        for (int m = 0; m < NV; m++) usp[m] = p.us[m * plane + o];
       }
     }
 
@@ -355,7 +362,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             if constexpr (STAGE == 1) {
 #pragma unroll
               for (int m = 0; m < NV; m++) {
-                double uu = p.us[m * plane + o];
+                double uu;
+                if constexpr (PREF) uu = usp[m];
+                else uu = p.us[m * plane + o];
                 if (m == 8) uu = glm_f * uu;
                 o9[m] = uu + dtx * d[m];  // rk_tvd21 / rk_std21
               }
