@@ -455,6 +455,7 @@ static Geo geo_of(const omhd_handle *h) {
 extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every, omhd_adjoint **out) {
   if (!h || !out) return set_error(OMHD_ERR_ARG, "null argument");
   if (nsteps < 1 || every < 1) return set_error(OMHD_ERR_ARG, "need nsteps >= 1 and every >= 1", nsteps, every);
+  if (h->cfg.flux_type != 3) return set_error(OMHD_ERR_TYPE, "the adjoint is built for the HLLD flux", h->cfg.flux_type);
   if (every > nsteps) every = nsteps;
   omhd_adjoint *a = new omhd_adjoint();
   a->h = h;

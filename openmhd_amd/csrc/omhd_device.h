@@ -296,6 +296,210 @@ __device__ __forceinline__ void hlld(const double L[NV], const double R[NV], dou
 }
 
 // ---------------------------------------------------------------------------
+// The other flux types of flux_solver.f90 at one interface (the type is
+// uniform over a launch): 0 LLF / Rusanov (:89-148), 1 HLL (:153-221),
+// 2 HLLC (:226-347), 4 LHLLD (:590-789).  Statements in the reference's order.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void side_state(const double S[NV], double U[4], double Fs[NV], double &B2, double &vB,
+                                           double &v2) {
+  constexpr int vn = D, vt1 = (D + 1) % 3, vt2 = (D + 2) % 3;
+  constexpr int bn = 5 + vn, bt1 = 5 + vt1, bt2 = 5 + vt2;
+  constexpr int RO = 4, PR = 3, EN = 3;
+  v2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+  B2 = S[5] * S[5] + S[6] * S[6] + S[7] * S[7];
+  vB = S[0] * S[5] + S[1] * S[6] + S[2] * S[7];
+  U[0] = S[RO] * S[0];
+  U[1] = S[RO] * S[1];
+  U[2] = S[RO] * S[2];
+  U[EN] = 0.5 * (S[RO] * v2 + B2) + R1 * S[PR];
+  Fs[RO] = S[RO] * S[vn];
+  Fs[vn] = S[RO] * S[vn] * S[vn] - S[bn] * S[bn] + S[PR] + 0.5 * B2;
+  Fs[vt1] = S[RO] * S[vn] * S[vt1] - S[bn] * S[bt1];
+  Fs[vt2] = S[RO] * S[vn] * S[vt2] - S[bn] * S[bt2];
+  Fs[EN] = (0.5 * S[RO] * v2 + R2 * S[PR] + B2) * S[vn] - vB * S[bn];
+  Fs[bt1] = S[vn] * S[bt1] - S[bn] * S[vt1];
+  Fs[bt2] = S[vn] * S[bt2] - S[bn] * S[vt2];
+  Fs[bn] = 0.0;
+  Fs[8] = 0.0;
+}
+
+// fast-mode speed^2 of one side, with the max(., 0) guard; with rv2 in place
+// of gamma p it is LHLLD's second estimate
+__device__ __forceinline__ double fast2(double f1, double B2, double bnv, double ro) {
+  const double f2 = 4 * (bnv * bnv);
+  return ((f1 + B2) + __builtin_sqrt(dmax((f1 + B2) * (f1 + B2) - f1 * f2, 0.0))) / (2 * ro);
+}
+
+template <int D>
+__device__ __forceinline__ void riemann_other(int type, const double L[NV], const double R[NV], double ch, double ch2,
+                                              double F[NV]) {
+  constexpr int vn = D, vt1 = (D + 1) % 3, vt2 = (D + 2) % 3;
+  constexpr int bn = 5 + vn, bt1 = 5 + vt1, bt2 = 5 + vt2;
+  constexpr int RO = 4, PR = 3, EN = 3;
+  double UL[4], UR[4], FL[NV], FR[NV], B2L, vBL, v2L, B2R, vBR, v2R;
+  side_state<D>(L, UL, FL, B2L, vBL, v2L);
+  side_state<D>(R, UR, FR, B2R, vBR, v2R);
+#pragma unroll
+  for (int m = 0; m < NV; m++) F[m] = 0.0;
+  if (type == 0) {  // LLF: no guard under the inner sqrt (:112-113)
+    double f1 = GAM * L[PR], f2 = 4 * (L[bn] * L[bn]);
+    const double aL = __builtin_sqrt(((f1 + B2L) + __builtin_sqrt((f1 + B2L) * (f1 + B2L) - f1 * f2)) / (2 * L[RO]));
+    f1 = GAM * R[PR];
+    f2 = 4 * (R[bn] * R[bn]);
+    const double aR = __builtin_sqrt(((f1 + B2R) + __builtin_sqrt((f1 + B2R) * (f1 + B2R) - f1 * f2)) / (2 * R[RO]));
+    f1 = dmax(__builtin_fabs(L[vn]) + aL, __builtin_fabs(R[vn]) + aR);
+#pragma unroll
+    for (int m = 0; m < 4; m++) F[m] = 0.5 * (FL[m] + FR[m] - f1 * (UR[m] - UL[m]));
+    F[RO] = 0.5 * (FL[RO] + FR[RO] - f1 * (R[RO] - L[RO]));
+    F[bt1] = 0.5 * (FL[bt1] + FR[bt1] - f1 * (R[bt1] - L[bt1]));
+    F[bt2] = 0.5 * (FL[bt2] + FR[bt2] - f1 * (R[bt2] - L[bt2]));
+  } else if (type == 1 || type == 2) {  // HLL, HLLC: fan speeds clamped at 0 (:207-208)
+    double aL = fast2(GAM * L[PR], B2L, L[bn], L[RO]), aR = fast2(GAM * R[PR], B2R, R[bn], R[RO]);
+    const double ptL = L[PR] + 0.5 * B2L, ptR = R[PR] + 0.5 * B2R;
+    double f1 = __builtin_sqrt(dmax(aL, aR));
+    aL = dmin(dmin(L[vn], R[vn]) - f1, 0.0);
+    aR = dmax(dmax(L[vn], R[vn]) + f1, 0.0);
+    f1 = 1.0 / (aR - aL);
+    double f2 = aL * aR;
+    F[bt1] = f1 * (aR * FL[bt1] - aL * FR[bt1] + f2 * (R[bt1] - L[bt1]));
+    F[bt2] = f1 * (aR * FL[bt2] - aL * FR[bt2] + f2 * (R[bt2] - L[bt2]));
+    if (type == 1) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) F[m] = f1 * (aR * FL[m] - aL * FR[m] + f2 * (UR[m] - UL[m]));
+      F[RO] = f1 * (aR * FL[RO] - aL * FR[RO] + f2 * (R[RO] - L[RO]));
+    } else {
+      double Uh[NV];
+#pragma unroll
+      for (int m = 0; m < 3; m++) Uh[m] = f1 * (aR * UR[m] - aL * UL[m] - FR[m] + FL[m]);
+#pragma unroll
+      for (int m = RO; m <= 7; m++) Uh[m] = f1 * (aR * R[m] - aL * L[m] - FR[m] + FL[m]);
+      f2 = 1 / Uh[RO];
+      const double aM = Uh[vn] * f2, at1 = Uh[vt1] * f2, at2 = Uh[vt2] * f2;
+      const double pt = ptL + L[RO] * (aL - L[vn]) * (aM - L[vn]);
+      const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
+      const double enL = ((aL - L[vn]) * UL[EN] - ptL * L[vn] + pt * aM +
+                          Uh[bn] * (vBL - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2])) / (aL - aM);
+      const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
+      const double enR = ((aR - R[vn]) * UR[EN] - ptR * R[vn] + pt * aM +
+                          Uh[bn] * (vBR - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2])) / (aR - aM);
+      const double wl = 0.5 + fsign(0.5, aM);
+      const double wr = 1.0 - wl;
+      F[vn] = wl * (aL * (roL * aM - UL[vn]) + FL[vn]) + wr * (aR * (roR * aM - UR[vn]) + FR[vn]);
+      F[vt1] = wl * (aL * (roL * at1 - UL[vt1]) + FL[vt1]) + wr * (aR * (roR * at1 - UR[vt1]) + FR[vt1]);
+      F[vt2] = wl * (aL * (roL * at2 - UL[vt2]) + FL[vt2]) + wr * (aR * (roR * at2 - UR[vt2]) + FR[vt2]);
+      F[EN] = wl * (aL * (enL - UL[EN]) + FL[EN]) + wr * (aR * (enR - UR[EN]) + FR[EN]);
+      F[RO] = wl * (aL * (roL - L[RO]) + FL[RO]) + wr * (aR * (roR - R[RO]) + FR[RO]);
+    }
+  } else {  // LHLLD (type 4)
+    double aL = fast2(GAM * L[PR], B2L, L[bn], L[RO]), aR = fast2(GAM * R[PR], B2R, R[bn], R[RO]);
+    const double ptL = L[PR] + 0.5 * B2L, ptR = R[PR] + 0.5 * B2R;
+    double aL1 = fast2(L[RO] * v2L, B2L, L[bn], L[RO]), aR1 = fast2(R[RO] * v2R, B2R, R[bn], R[RO]);
+    double f1 = dmin(1.0, dmax(aL1, aR1) / dmax(aL, aR));  // chi^2
+    const double psi = 2 * __builtin_sqrt(f1) - f1;
+    f1 = __builtin_sqrt(dmax(aL, aR));
+    aL = dmin(L[vn], R[vn]) - f1;
+    aR = dmax(L[vn], R[vn]) + f1;
+    if (aL >= 0) {
+#pragma unroll
+      for (int m = 0; m < NV; m++) F[m] = FL[m];
+    } else if (aR <= 0) {
+#pragma unroll
+      for (int m = 0; m < NV; m++) F[m] = FR[m];
+    } else {
+      double Uh[NV], UL1[NV], UR1[NV];
+      f1 = 1.0 / (aR - aL);
+#pragma unroll
+      for (int m = 0; m < 3; m++) Uh[m] = f1 * (aR * UR[m] - aL * UL[m] - FR[m] + FL[m]);
+#pragma unroll
+      for (int m = RO; m <= 7; m++) Uh[m] = f1 * (aR * R[m] - aL * L[m] - FR[m] + FL[m]);
+      const double aM = Uh[vn] / Uh[RO];
+      const double pt = (R[RO] * ptL * (aR - R[vn]) - L[RO] * ptR * (aL - L[vn]) +
+                         psi * L[RO] * R[RO] * (aR - R[vn]) * (aL - L[vn]) * (R[vn] - L[vn])) /
+                        (R[RO] * (aR - R[vn]) - L[RO] * (aL - L[vn]));
+      const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
+      const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
+      const double sroL = __builtin_sqrt(roL), sroR = __builtin_sqrt(roR);
+      const double aVL = __builtin_fabs(Uh[bn]) / sroL;
+      const double aVR = __builtin_fabs(Uh[bn]) / sroR;
+      const double hb = Uh[bn], hb2 = hb * hb;
+      double fd = 1.0 / (L[RO] * (aL - L[vn]) * (aL - aM) - hb2);
+      const double dL = aL - L[vn];
+      UL1[bt1] = L[bt1] * fd * (L[RO] * (dL * dL) - hb2);
+      UL1[bt2] = L[bt2] * fd * (L[RO] * (dL * dL) - hb2);
+      const double vt1L = L[vt1] - fd * hb * L[bt1] * (aM - L[vn]);
+      const double vt2L = L[vt2] - fd * hb * L[bt2] * (aM - L[vn]);
+      UL1[RO] = roL;
+      UL1[EN] = ((aL - L[vn]) * UL[EN] - ptL * L[vn] + pt * aM +
+                 hb * (vBL - aM * hb - vt1L * UL1[bt1] - vt2L * UL1[bt2])) / (aL - aM);
+      UL1[vn] = roL * aM;
+      UL1[vt1] = roL * vt1L;
+      UL1[vt2] = roL * vt2L;
+      fd = 1.0 / (R[RO] * (aR - R[vn]) * (aR - aM) - hb2);
+      const double dR = aR - R[vn];
+      UR1[bt1] = R[bt1] * fd * (R[RO] * (dR * dR) - hb2);
+      UR1[bt2] = R[bt2] * fd * (R[RO] * (dR * dR) - hb2);
+      const double vt1R = R[vt1] - fd * hb * R[bt1] * (aM - R[vn]);
+      const double vt2R = R[vt2] - fd * hb * R[bt2] * (aM - R[vn]);
+      UR1[RO] = roR;
+      UR1[EN] = ((aR - R[vn]) * UR[EN] - ptR * R[vn] + pt * aM +
+                 hb * (vBR - aM * hb - vt1R * UR1[bt1] - vt2R * UR1[bt2])) / (aR - aM);
+      UR1[vn] = roR * aM;
+      UR1[vt1] = roR * vt1R;
+      UR1[vt2] = roR * vt2R;
+      aL1 = aM - aVL;
+      aR1 = aM + aVR;
+      if (aL1 >= 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) F[m] = aL * (UL1[m] - UL[m]) + FL[m];
+        F[RO] = aL * (UL1[RO] - L[RO]) + FL[RO];
+        F[bt1] = aL * (UL1[bt1] - L[bt1]) + FL[bt1];
+        F[bt2] = aL * (UL1[bt2] - L[bt2]) + FL[bt2];
+      } else if (aR1 <= 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) F[m] = aR * (UR1[m] - UR[m]) + FR[m];
+        F[RO] = aR * (UR1[RO] - R[RO]) + FR[RO];
+        F[bt1] = aR * (UR1[bt1] - R[bt1]) + FR[bt1];
+        F[bt2] = aR * (UR1[bt2] - R[bt2]) + FR[bt2];
+      } else {
+        const double fs = 1.0 / (sroL + sroR);
+        const double sg = fsign(1.0, hb);
+        const double at1 = fs * (sroL * vt1L + sroR * vt1R + (UR1[bt1] - UL1[bt1]) * sg);
+        const double at2 = fs * (sroL * vt2L + sroR * vt2R + (UR1[bt2] - UL1[bt2]) * sg);
+        double U2[NV];
+        U2[bt1] = fs * (sroL * UR1[bt1] + sroR * UL1[bt1] + sroL * sroR * (vt1R - vt1L) * sg);
+        U2[bt2] = fs * (sroL * UR1[bt2] + sroR * UL1[bt2] + sroL * sroR * (vt2R - vt2L) * sg);
+        if (aM >= 0) {
+          U2[RO] = roL;
+          U2[EN] = UL1[EN] - sroL * (vt1L * UL1[bt1] + vt2L * UL1[bt2] - at1 * U2[bt1] - at2 * U2[bt2]) * sg;
+          U2[vn] = roL * aM;
+          U2[vt1] = roL * at1;
+          U2[vt2] = roL * at2;
+#pragma unroll
+          for (int m = 0; m < 4; m++) F[m] = aL1 * U2[m] - (aL1 - aL) * UL1[m] - aL * UL[m] + FL[m];
+          F[RO] = aL1 * U2[RO] - (aL1 - aL) * UL1[RO] - aL * L[RO] + FL[RO];
+          F[bt1] = aL1 * U2[bt1] - (aL1 - aL) * UL1[bt1] - aL * L[bt1] + FL[bt1];
+          F[bt2] = aL1 * U2[bt2] - (aL1 - aL) * UL1[bt2] - aL * L[bt2] + FL[bt2];
+        } else {
+          U2[RO] = roR;
+          U2[EN] = UR1[EN] + sroR * (vt1R * UR1[bt1] + vt2R * UR1[bt2] - at1 * U2[bt1] - at2 * U2[bt2]) * sg;
+          U2[vn] = roR * aM;
+          U2[vt1] = roR * at1;
+          U2[vt2] = roR * at2;
+#pragma unroll
+          for (int m = 0; m < 4; m++) F[m] = aR1 * U2[m] - (aR1 - aR) * UR1[m] - aR * UR[m] + FR[m];
+          F[RO] = aR1 * U2[RO] - (aR1 - aR) * UR1[RO] - aR * R[RO] + FR[RO];
+          F[bt1] = aR1 * U2[bt1] - (aR1 - aR) * UR1[bt1] - aR * R[bt1] + FR[bt1];
+          F[bt2] = aR1 * U2[bt2] - (aR1 - aR) * UR1[bt2] - aR * R[bt2] + FR[bt2];
+        }
+      }
+    }
+  }
+  F[bn] = 0.5 * ((L[8] + R[8]) - ch * (R[bn] - L[bn]));
+  F[8] = 0.5 * (ch2 * (L[bn] + R[bn]) - ch * (R[8] - L[8]));
+}
+
+// ---------------------------------------------------------------------------
 // Fastest magnetosonic signal speed of one cell, set_dt.f90:27-45 (2D) and
 // 3D_basic/set_dt.f90 (ND == 3).  u: conserved, w: primitives from u.
 // ---------------------------------------------------------------------------

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_limiter(const double *__restrict__ wk, 
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ __launch_bounds__(256) void k_flux(double *__restrict__ F, const double *__restrict__ VL, const double *__restrict__ VR, double ch,
-                       int ix, int jx, int kx, int i0, int ni, int j0, int nj, int k0, int nk) {
+                       int ix, int jx, int kx, int i0, int ni, int j0, int nj, int k0, int nk, int type) {
   size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (size_t)ni * nj * nk) return;
   const int i = i0 + (int)(gid % ni);
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void k_flux(double *__restrict__ F, const doub
     L[m] = VL[m * n + c];
     R[m] = VR[m * n + c];
   }
-  hlld<D>(L, R, ch, ch * ch, f);
+  if (type == 3) hlld<D>(L, R, ch, ch * ch, f);
+  else riemann_other<D>(type, L, R, ch, ch * ch, f);
 #pragma unroll
   for (int m = 0; m < NV; m++) F[m * n + c] = f[m];
 }
@@ -366,7 +367,7 @@ extern "C" int omhd_flux_solver(double *F, const double *VL, const double *VR, d
                                 int32_t kx, int32_t dir, int32_t type) {
   if (!F || !VL || !VR) return set_error(OMHD_ERR_ARG, "null array");
   if (int rc = check_ext(ix, jx, kx)) return rc;
-  if (type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", type);
+  if (type < 0 || type > 4) return set_error(OMHD_ERR_TYPE, "unknown flux_type", type);
   if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
   hipStream_t s = dropin_stream();
   const size_t n = (size_t)ix * jx * kx;
@@ -384,9 +385,9 @@ extern "C" int omhd_flux_solver(double *F, const double *VL, const double *VR, d
   if (dir == 3) { k0 = 0; nk = kx - 1; }
   const size_t total = (size_t)(ni > 0 ? ni : 0) * nj * nk;
   if (total > 0) {
-    if (dir == 1) k_flux<0><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
-    if (dir == 2) k_flux<1><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
-    if (dir == 3) k_flux<2><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk);
+    if (dir == 1) k_flux<0><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk, type);
+    if (dir == 2) k_flux<1><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk, type);
+    if (dir == 3) k_flux<2><<<nblocks(total), TPB, 0, s>>>(f.dev, l.dev, r.dev, ch, ix, jx, kx, i0, ni, j0, nj, k0, nk, type);
     OMHD_HIP(hipGetLastError());
   }
   if (int rc = f.finish(s)) return rc;

@@ -61,6 +61,7 @@ struct Sweep {
   int *err;                       // [code, i, j, k] of the first failure
   const double *dpar;  // device-side step parameters (DevPar) or null: use the fields above
   double *U, *U1;      // ensemble members: their two state buffers
+  int ftype;           // flux type of the generic-solver kernels (0 LLF, 1 HLL, 2 HLLC, 4 LHLLD)
 };
 
 // Step parameters computed on the device from the reduced CFL speed (device
@@ -130,8 +131,9 @@ using LdsOf = typename std::conditional<RES != 0, SweepLdsRes, SweepLds>::type;
 // One tile: strip x chunk (x slice) of one grid.
 // PREF: stage 2 issues the U^n loads of the combined row early (where the
 // register budget allows it without spills: the resident kernels, not the
-// ensemble kernel, whose per-member parameters take more registers)
-template <int LIM, int STAGE, int TT, int PASS, int RES, bool PREF = true>
+// ensemble kernel, whose per-member parameters take more registers).
+// GEN: the flux is p.ftype's (LLF, HLL, HLLC, LHLLD) instead of HLLD.
+template <int LIM, int STAGE, int TT, int PASS, int RES, bool PREF = true, bool GEN = false>
 __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RES> &lds) {
   constexpr bool XS = PASS != PASS_Z;         // x-sweep of the oldest row
   constexpr bool FINAL = PASS != PASS_XY;     // RK update (else write acc)
@@ -277,7 +279,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
 #pragma unroll
           for (int m = 0; m < NV; m++) Rx[m] = flip(Lx[m], p.shi[0], m);
         }
-        hlld<0>(Lx, Rx, ch, ch2, F);
+        if constexpr (GEN) riemann_other<0>(p.ftype, Lx, Rx, ch, ch2, F);
+        else hlld<0>(Lx, Rx, ch, ch2, F);
         if constexpr (RES != 0) {  // flux_resistive.cuf:89-101 at x-face (c+1/2, row rn-2, plane slice)
           const int l = lane + 1;
           const double *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
@@ -319,7 +322,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
           }
         }
         double G[NV];
-        hlld<MD>(Lf, Rf, ch, ch2, G);
+        if constexpr (GEN) riemann_other<MD>(p.ftype, Lf, Rf, ch, ch2, G);
+        else hlld<MD>(Lf, Rf, ch, ch2, G);
         if constexpr (RES != 0) {
           const int l = lane + 1;
           const double *A5 = lds.ring[sa][5], *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
@@ -408,6 +412,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
   march_tile<LIM, STAGE, TT, PASS, RES, (LIM != 0)>(p, t, lds);
+}
+
+// 2D step with one of the other flux types (p.ftype), same structure
+template <int LIM, int STAGE, int TT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march_gen(const Sweep p) {
+  __shared__ LdsOf<0> lds;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  if (t >= p.ntiles) return;
+  march_tile<LIM, STAGE, TT, PASS_2D, 0, false, true>(p, t, lds);
 }
 
 // Ensemble: one persistent launch per RK stage for every member.  The tile
@@ -553,7 +566,22 @@ march_fn pick2(int lim, int stage, int tt) {
   }
 }
 
-march_fn march_kernel(int pass, int res, int lim, int stage, int tt) {
+template <int LIM>
+march_fn pick_gen(int stage, int tt) {
+  if (stage == 1) return tt == 0 ? k_march_gen<LIM, 1, 0> : k_march_gen<LIM, 1, 1>;
+  return tt == 0 ? k_march_gen<LIM, 2, 0> : k_march_gen<LIM, 2, 1>;
+}
+
+march_fn march_kernel(int pass, int res, int lim, int stage, int tt, bool gen = false) {
+  if (pass == PASS_2D && gen) {
+    switch (lim) {
+      case 0: return pick_gen<0>(stage, tt);
+      case 1: return pick_gen<1>(stage, tt);
+      case 2: return pick_gen<2>(stage, tt);
+      case 3: return pick_gen<3>(stage, tt);
+      default: return pick_gen<4>(stage, tt);
+    }
+  }
   if (pass == PASS_2D) return pick2<PASS_2D, 0>(lim, stage, tt);
   if (pass == PASS_XY) return res ? pick2<PASS_XY, 1>(lim, stage, tt) : pick2<PASS_XY, 0>(lim, stage, tt);
   return res ? pick2<PASS_Z, 1>(lim, stage, tt) : pick2<PASS_Z, 0>(lim, stage, tt);
@@ -612,7 +640,9 @@ extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
   if (cfg->nx < 2 || cfg->ny < 2 || (cfg->ndim == 3 && nz < 2))
     return set_error(OMHD_ERR_EXTENT, "need nx, ny (, nz) >= 2", cfg->nx, cfg->ny, nz);
   if (cfg->lm_type < 0 || cfg->lm_type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", cfg->lm_type);
-  if (cfg->flux_type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", cfg->flux_type);
+  if (cfg->flux_type < 0 || cfg->flux_type > 4) return set_error(OMHD_ERR_TYPE, "unknown flux_type", cfg->flux_type);
+  if (cfg->flux_type != 3 && (cfg->ndim != 2 || cfg->resistive))
+    return set_error(OMHD_ERR_TYPE, "flux types other than HLLD are built for the 2D step", cfg->flux_type);
   if (cfg->time_type != 0 && cfg->time_type != 1) return set_error(OMHD_ERR_TYPE, "unknown time_type", cfg->time_type);
   if (!(cfg->dx > 0) || !(cfg->cfl > 0)) return set_error(OMHD_ERR_ARG, "dx and cfl must be positive");
   if (cfg->resistive && cfg->ndim != 3)
@@ -800,7 +830,7 @@ int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, 
   const int res = c.resistive ? 1 : 0;
   Sweep pa = base_sweep(h, three ? PASS_XY : PASS_2D);
   Sweep pz = three ? base_sweep(h, PASS_Z) : pa;
-  const march_fn a = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, stage, c.time_type);
+  const march_fn a = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, stage, c.time_type, c.flux_type != 3);
   const march_fn z = three ? march_kernel(PASS_Z, res, c.lm_type, stage, c.time_type) : nullptr;
   const double cr = 0.2;                          // glm_ss2.f90:18
   const double glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
@@ -814,6 +844,7 @@ int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, 
     q->u0 = u0;
     q->out = out;
     q->dtx = dtx;
+    q->ftype = c.flux_type;
   }
   hipLaunchKernelGGL(a, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
   OMHD_HIP(hipGetLastError());
@@ -1040,7 +1071,7 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
     const omhd_config &c = cfgs[i];
     if (c.ndim != 2) return set_error(OMHD_ERR_ARG, "ensemble members are 2D grids", i);
     if (c.nx < 2 || c.ny < 2) return set_error(OMHD_ERR_EXTENT, "need nx, ny >= 2", c.nx, c.ny, i);
-    if (c.flux_type != 3) return set_error(OMHD_ERR_TYPE, "only flux_type 3 (HLLD) is built", i);
+    if (c.flux_type != 3) return set_error(OMHD_ERR_TYPE, "the ensemble is built for the HLLD flux", i);
     if (c.lm_type != cfgs[0].lm_type || c.time_type != cfgs[0].time_type)
       return set_error(OMHD_ERR_ARG, "members share one limiter and one time type (one kernel per stage)", i);
     if (c.lm_type < 0 || c.lm_type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", c.lm_type);

@@ -19,7 +19,8 @@
  * Parity status per function (what pins it):
  *   or_u2v            flang vectors + closed form (uniform state)
  *   or_limiter        flang vectors incl. -0.0 and sign-mismatch cases
- *   or_flux_solver    flang vectors for every HLLD branch x dir 1/2/3
+ *   or_flux_solver    flang vectors for every HLLD branch x dir 1/2/3, and
+ *                     for the LLF, HLL, HLLC and LHLLD types x dir 1/2/3
  *   or_glm_ss2        flang vectors (single multiply)
  *   or_rk_*           flang vectors
  *   or_set_dt         flang vectors
@@ -375,6 +376,212 @@ void or_hlld(const double *L, const double *R, double ch, int d, double *F, int 
   if (branch) *branch = br;
 }
 
+/* ------------------------------------------------------------------------
+ * The other flux types of flux_solver.f90 at one interface: 0 LLF (Rusanov,
+ * :89-148), 1 HLL (:153-221), 2 HLLC (:226-347), 4 LHLLD (:590-789).
+ * ---------------------------------------------------------------------- */
+
+/* one side: conserved momenta/energy, physical flux, v.v, B.B, v.B
+ * (flux_solver.f90:93-108 and the same statements of every type) */
+static void or_side(const double *S, int vn, int vt1, int vt2, int bn, int bt1, int bt2, double *U, double *Fs,
+                    double *v2o, double *B2o, double *vBo)
+{
+  const double r1 = 1.0 / (gam - 1.0), r2 = gam / (gam - 1.0);
+  const int RO = 4, PR = 3, EN = 3;
+  double v2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+  double B2 = S[5] * S[5] + S[6] * S[6] + S[7] * S[7];
+  double vB = S[0] * S[5] + S[1] * S[6] + S[2] * S[7];
+  U[0] = S[RO] * S[0];
+  U[1] = S[RO] * S[1];
+  U[2] = S[RO] * S[2];
+  U[EN] = 0.5 * (S[RO] * v2 + B2) + r1 * S[PR];
+  for (int m = 0; m < 9; m++) Fs[m] = 0.0;
+  Fs[RO] = S[RO] * S[vn];
+  Fs[vn] = S[RO] * S[vn] * S[vn] - S[bn] * S[bn] + S[PR] + 0.5 * B2;
+  Fs[vt1] = S[RO] * S[vn] * S[vt1] - S[bn] * S[bt1];
+  Fs[vt2] = S[RO] * S[vn] * S[vt2] - S[bn] * S[bt2];
+  Fs[EN] = (0.5 * S[RO] * v2 + r2 * S[PR] + B2) * S[vn] - vB * S[bn];
+  Fs[bt1] = S[vn] * S[bt1] - S[bn] * S[vt1];
+  Fs[bt2] = S[vn] * S[bt2] - S[bn] * S[vt2];
+  *v2o = v2;
+  *B2o = B2;
+  *vBo = vB;
+}
+
+/* fast-mode speed^2 with the max(.,0) guard (types 1-4) */
+static double or_fast2(const double *S, double B2, int bn)
+{
+  double f1 = gam * S[3];
+  double f2 = 4 * (S[bn] * S[bn]);
+  return ((f1 + B2) + sqrt(fmax2((f1 + B2) * (f1 + B2) - f1 * f2, 0.0))) / (2 * S[4]);
+}
+
+void or_riemann(int type, const double *L, const double *R, double ch, int d, double *F)
+{
+  if (type == 3) {
+    or_hlld(L, R, ch, d, F, NULL);
+    return;
+  }
+  const int vn = d % 3, vt1 = (d + 1) % 3, vt2 = (d + 2) % 3;
+  const int mn = vn, mt1 = vt1, mt2 = vt2;
+  const int bn = 5 + vn, bt1 = 5 + vt1, bt2 = 5 + vt2;
+  const int RO = 4, EN = 3, PS = 8;
+  double UL[4], UR[4], FL[9], FR[9], v2L, B2L, vBL, v2R, B2R, vBR;
+  or_side(L, vn, vt1, vt2, bn, bt1, bt2, UL, FL, &v2L, &B2L, &vBL);
+  or_side(R, vn, vt1, vt2, bn, bt1, bt2, UR, FR, &v2R, &B2R, &vBR);
+  for (int m = 0; m < 9; m++) F[m] = 0.0;
+  if (type == 0) { /* LLF (:110-143) */
+    double f1 = gam * L[3], f2 = 4 * (L[bn] * L[bn]);
+    double aL = sqrt(((f1 + B2L) + sqrt((f1 + B2L) * (f1 + B2L) - f1 * f2)) / (2 * L[RO]));
+    f1 = gam * R[3];
+    f2 = 4 * (R[bn] * R[bn]);
+    double aR = sqrt(((f1 + B2R) + sqrt((f1 + B2R) * (f1 + B2R) - f1 * f2)) / (2 * R[RO]));
+    f1 = fmax2(fabs(L[vn]) + aL, fabs(R[vn]) + aR);
+    for (int m = 0; m < 4; m++) F[m] = 0.5 * (FL[m] + FR[m] - f1 * (UR[m] - UL[m]));
+    F[RO] = 0.5 * (FL[RO] + FR[RO] - f1 * (R[RO] - L[RO]));
+    F[bt1] = 0.5 * (FL[bt1] + FR[bt1] - f1 * (R[bt1] - L[bt1]));
+    F[bt2] = 0.5 * (FL[bt2] + FR[bt2] - f1 * (R[bt2] - L[bt2]));
+  } else if (type == 1 || type == 2) { /* HLL (:172-216), HLLC (:247-342) */
+    double aL = or_fast2(L, B2L, bn), aR = or_fast2(R, B2R, bn);
+    const double ptL = L[3] + 0.5 * B2L, ptR = R[3] + 0.5 * B2R;
+    double f1 = sqrt(fmax2(aL, aR));
+    aL = fmin2(fmin2(L[vn], R[vn]) - f1, 0.0);
+    aR = fmax2(fmax2(L[vn], R[vn]) + f1, 0.0);
+    f1 = 1.0 / (aR - aL);
+    double f2 = aL * aR;
+    if (type == 1) {
+      for (int m = 0; m < 4; m++) F[m] = f1 * (aR * FL[m] - aL * FR[m] + f2 * (UR[m] - UL[m]));
+      F[RO] = f1 * (aR * FL[RO] - aL * FR[RO] + f2 * (R[RO] - L[RO]));
+    } else {
+      double Uh[9];
+      for (int m = 0; m < 3; m++) Uh[m] = f1 * (aR * UR[m] - aL * UL[m] - FR[m] + FL[m]);
+      for (int m = RO; m <= 7; m++) Uh[m] = f1 * (aR * R[m] - aL * L[m] - FR[m] + FL[m]);
+      F[bt1] = f1 * (aR * FL[bt1] - aL * FR[bt1] + f2 * (R[bt1] - L[bt1]));
+      F[bt2] = f1 * (aR * FL[bt2] - aL * FR[bt2] + f2 * (R[bt2] - L[bt2]));
+      f2 = 1 / Uh[RO];
+      const double aM = Uh[mn] * f2, at1 = Uh[mt1] * f2, at2 = Uh[mt2] * f2;
+      const double pt = ptL + L[RO] * (aL - L[vn]) * (aM - L[vn]);
+      const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
+      const double enL = ((aL - L[vn]) * UL[EN] - ptL * L[vn] + pt * aM +
+                          Uh[bn] * (vBL - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2])) / (aL - aM);
+      const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
+      const double enR = ((aR - R[vn]) * UR[EN] - ptR * R[vn] + pt * aM +
+                          Uh[bn] * (vBR - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2])) / (aR - aM);
+      f1 = 0.5 + fsign(0.5, aM);
+      f2 = 1.0 - f1;
+      F[mn] = f1 * (aL * (roL * aM - UL[mn]) + FL[mn]) + f2 * (aR * (roR * aM - UR[mn]) + FR[mn]);
+      F[mt1] = f1 * (aL * (roL * at1 - UL[mt1]) + FL[mt1]) + f2 * (aR * (roR * at1 - UR[mt1]) + FR[mt1]);
+      F[mt2] = f1 * (aL * (roL * at2 - UL[mt2]) + FL[mt2]) + f2 * (aR * (roR * at2 - UR[mt2]) + FR[mt2]);
+      F[EN] = f1 * (aL * (enL - UL[EN]) + FL[EN]) + f2 * (aR * (enR - UR[EN]) + FR[EN]);
+      F[RO] = f1 * (aL * (roL - L[RO]) + FL[RO]) + f2 * (aR * (roR - R[RO]) + FR[RO]);
+    }
+    if (type == 1) {
+      F[bt1] = f1 * (aR * FL[bt1] - aL * FR[bt1] + f2 * (R[bt1] - L[bt1]));
+      F[bt2] = f1 * (aR * FL[bt2] - aL * FR[bt2] + f2 * (R[bt2] - L[bt2]));
+    }
+  } else { /* LHLLD (:590-789) */
+    double aL = or_fast2(L, B2L, bn), aR = or_fast2(R, B2R, bn);
+    const double ptL = L[3] + 0.5 * B2L, ptR = R[3] + 0.5 * B2R;
+    /* the second fast-mode estimate uses rho v^2 in place of gamma p */
+    double f1 = L[RO] * v2L, f2 = 4 * (L[bn] * L[bn]);
+    double aL1 = ((f1 + B2L) + sqrt(fmax2((f1 + B2L) * (f1 + B2L) - f1 * f2, 0.0))) / (2 * L[RO]);
+    f1 = R[RO] * v2R;
+    f2 = 4 * (R[bn] * R[bn]);
+    double aR1 = ((f1 + B2R) + sqrt(fmax2((f1 + B2R) * (f1 + B2R) - f1 * f2, 0.0))) / (2 * R[RO]);
+    f1 = fmin2(1.0, fmax2(aL1, aR1) / fmax2(aL, aR)); /* chi^2 */
+    f2 = 2 * sqrt(f1) - f1;                            /* psi */
+    const double psi = f2;
+    f1 = sqrt(fmax2(aL, aR));
+    aL = fmin2(L[vn], R[vn]) - f1;
+    aR = fmax2(L[vn], R[vn]) + f1;
+    if (aL >= 0) {
+      for (int m = 0; m < 9; m++) F[m] = FL[m];
+    } else if (aR <= 0) {
+      for (int m = 0; m < 9; m++) F[m] = FR[m];
+    } else {
+      double Uh[9], UL1[9], UR1[9], U2[9];
+      f1 = 1.0 / (aR - aL);
+      for (int m = 0; m < 3; m++) Uh[m] = f1 * (aR * UR[m] - aL * UL[m] - FR[m] + FL[m]);
+      for (int m = RO; m <= 7; m++) Uh[m] = f1 * (aR * R[m] - aL * L[m] - FR[m] + FL[m]);
+      const double aM = Uh[mn] / Uh[RO];
+      const double pt = (R[RO] * ptL * (aR - R[vn]) - L[RO] * ptR * (aL - L[vn]) +
+                         psi * L[RO] * R[RO] * (aR - R[vn]) * (aL - L[vn]) * (R[vn] - L[vn])) /
+                        (R[RO] * (aR - R[vn]) - L[RO] * (aL - L[vn]));
+      const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
+      const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
+      const double aVL = fabs(Uh[bn]) / sqrt(roL);
+      const double aVR = fabs(Uh[bn]) / sqrt(roR);
+      f1 = 1.0 / (L[RO] * (aL - L[vn]) * (aL - aM) - Uh[bn] * Uh[bn]);
+      UL1[bt1] = L[bt1] * f1 * (L[RO] * ((aL - L[vn]) * (aL - L[vn])) - Uh[bn] * Uh[bn]);
+      UL1[bt2] = L[bt2] * f1 * (L[RO] * ((aL - L[vn]) * (aL - L[vn])) - Uh[bn] * Uh[bn]);
+      const double vt1L = L[vt1] - f1 * Uh[bn] * L[bt1] * (aM - L[vn]);
+      const double vt2L = L[vt2] - f1 * Uh[bn] * L[bt2] * (aM - L[vn]);
+      UL1[RO] = roL;
+      UL1[EN] = ((aL - L[vn]) * UL[EN] - ptL * L[vn] + pt * aM +
+                 Uh[bn] * (vBL - aM * Uh[bn] - vt1L * UL1[bt1] - vt2L * UL1[bt2])) / (aL - aM);
+      UL1[mn] = roL * aM;
+      UL1[mt1] = roL * vt1L;
+      UL1[mt2] = roL * vt2L;
+      f1 = 1.0 / (R[RO] * (aR - R[vn]) * (aR - aM) - Uh[bn] * Uh[bn]);
+      UR1[bt1] = R[bt1] * f1 * (R[RO] * ((aR - R[vn]) * (aR - R[vn])) - Uh[bn] * Uh[bn]);
+      UR1[bt2] = R[bt2] * f1 * (R[RO] * ((aR - R[vn]) * (aR - R[vn])) - Uh[bn] * Uh[bn]);
+      const double vt1R = R[vt1] - f1 * Uh[bn] * R[bt1] * (aM - R[vn]);
+      const double vt2R = R[vt2] - f1 * Uh[bn] * R[bt2] * (aM - R[vn]);
+      UR1[RO] = roR;
+      UR1[EN] = ((aR - R[vn]) * UR[EN] - ptR * R[vn] + pt * aM +
+                 Uh[bn] * (vBR - aM * Uh[bn] - vt1R * UR1[bt1] - vt2R * UR1[bt2])) / (aR - aM);
+      UR1[mn] = roR * aM;
+      UR1[mt1] = roR * vt1R;
+      UR1[mt2] = roR * vt2R;
+      aL1 = aM - aVL;
+      aR1 = aM + aVR;
+      if (aL1 >= 0) {
+        for (int m = 0; m < 4; m++) F[m] = aL * (UL1[m] - UL[m]) + FL[m];
+        F[RO] = aL * (UL1[RO] - L[RO]) + FL[RO];
+        F[bt1] = aL * (UL1[bt1] - L[bt1]) + FL[bt1];
+        F[bt2] = aL * (UL1[bt2] - L[bt2]) + FL[bt2];
+      } else if (aR1 <= 0) {
+        for (int m = 0; m < 4; m++) F[m] = aR * (UR1[m] - UR[m]) + FR[m];
+        F[RO] = aR * (UR1[RO] - R[RO]) + FR[RO];
+        F[bt1] = aR * (UR1[bt1] - R[bt1]) + FR[bt1];
+        F[bt2] = aR * (UR1[bt2] - R[bt2]) + FR[bt2];
+      } else {
+        const double roLs = sqrt(roL), roRs = sqrt(roR);
+        f1 = 1.0 / (roLs + roRs);
+        f2 = fsign(1.0, Uh[bn]);
+        const double at1 = f1 * (roLs * vt1L + roRs * vt1R + (UR1[bt1] - UL1[bt1]) * f2);
+        const double at2 = f1 * (roLs * vt2L + roRs * vt2R + (UR1[bt2] - UL1[bt2]) * f2);
+        U2[bt1] = f1 * (roLs * UR1[bt1] + roRs * UL1[bt1] + roLs * roRs * (vt1R - vt1L) * f2);
+        U2[bt2] = f1 * (roLs * UR1[bt2] + roRs * UL1[bt2] + roLs * roRs * (vt2R - vt2L) * f2);
+        if (aM >= 0) {
+          U2[RO] = roL;
+          U2[EN] = UL1[EN] - roLs * (vt1L * UL1[bt1] + vt2L * UL1[bt2] - at1 * U2[bt1] - at2 * U2[bt2]) * f2;
+          U2[mn] = roL * aM;
+          U2[mt1] = roL * at1;
+          U2[mt2] = roL * at2;
+          for (int m = 0; m < 4; m++) F[m] = aL1 * U2[m] - (aL1 - aL) * UL1[m] - aL * UL[m] + FL[m];
+          F[RO] = aL1 * U2[RO] - (aL1 - aL) * UL1[RO] - aL * L[RO] + FL[RO];
+          F[bt1] = aL1 * U2[bt1] - (aL1 - aL) * UL1[bt1] - aL * L[bt1] + FL[bt1];
+          F[bt2] = aL1 * U2[bt2] - (aL1 - aL) * UL1[bt2] - aL * L[bt2] + FL[bt2];
+        } else {
+          U2[RO] = roR;
+          U2[EN] = UR1[EN] + roRs * (vt1R * UR1[bt1] + vt2R * UR1[bt2] - at1 * U2[bt1] - at2 * U2[bt2]) * f2;
+          U2[mn] = roR * aM;
+          U2[mt1] = roR * at1;
+          U2[mt2] = roR * at2;
+          for (int m = 0; m < 4; m++) F[m] = aR1 * U2[m] - (aR1 - aR) * UR1[m] - aR * UR[m] + FR[m];
+          F[RO] = aR1 * U2[RO] - (aR1 - aR) * UR1[RO] - aR * R[RO] + FR[RO];
+          F[bt1] = aR1 * U2[bt1] - (aR1 - aR) * UR1[bt1] - aR * R[bt1] + FR[bt1];
+          F[bt2] = aR1 * U2[bt2] - (aR1 - aR) * UR1[bt2] - aR * R[bt2] + FR[bt2];
+        }
+      }
+    }
+  }
+  /* GLM flux pair, every type (:141-142 and the same lines of each type) */
+  F[bn] = 0.5 * ((L[PS] + R[PS]) - ch * (R[bn] - L[bn]));
+  F[PS] = 0.5 * (ch * ch * (L[bn] + R[bn]) - ch * (R[PS] - L[PS]));
+}
+
 /* flux_solver(F,VL,VR,ch,ix,jx[,kx],dir,type): flux_solver.f90:1-81 frame.
  * The CPU build zeroes F first (:57); interfaces covered: along dir 1..n-1,
  * transverse min(2,.)..max(1,.-1). */
@@ -382,7 +589,7 @@ int or_flux_solver(double *F, const double *VL, const double *VR, double ch, int
                    int *branches)
 {
   if (dir < 1 || dir > 3) return OR_ERR_DIR;
-  if (type != 3) return OR_ERR_TYPE;
+  if (type < 0 || type > 4) return OR_ERR_TYPE;
   size_t n = (size_t)ix * jx * kx;
   memset(F, 0, sizeof(double) * 9 * n);
   int is = imin(2, ix), ie = imax(1, ix - 1), js = imin(2, jx), je = imax(1, jx - 1), ks = imin(2, kx), ke = imax(1, kx - 1);
@@ -396,8 +603,9 @@ int or_flux_solver(double *F, const double *VL, const double *VR, double ch, int
         size_t c = IX3(i, j, k);
         double L[9], R[9], f[9];
         for (int m = 0; m < 9; m++) { L[m] = VL[c + m * n]; R[m] = VR[c + m * n]; }
-        int br;
-        or_hlld(L, R, ch, dir - 1, f, &br);
+        int br = 0;
+        if (type == 3) or_hlld(L, R, ch, dir - 1, f, &br);
+        else or_riemann(type, L, R, ch, dir - 1, f);
         for (int m = 0; m < 9; m++) F[c + m * n] = f[m];
         if (branches) branches[c] = br;
       }

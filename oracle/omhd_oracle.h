@@ -48,6 +48,8 @@ typedef struct or_problem {
 int or_u2v(const double *U, double *V, int ix, int jx, int kx, long *bad);
 int or_limiter(const double *wk, double *wL, double *wR, int ix, int jx, int kx, int dir, int type);
 void or_hlld(const double *L, const double *R, double ch, int d, double *F, int *branch);
+/* flux types 0 LLF, 1 HLL, 2 HLLC, 3 HLLD, 4 LHLLD at one interface */
+void or_riemann(int type, const double *L, const double *R, double ch, int d, double *F);
 int or_flux_solver(double *F, const double *VL, const double *VR, double ch, int ix, int jx, int kx, int dir, int type,
                    int *branches);
 void or_glm_ss2(double *U, double ch, double dt, int ix, int jx, int kx);

@@ -6,7 +6,7 @@ subroutines on crafted inputs and stores inputs + outputs as .npz (plain
 arrays, no pickles).  Branch labels for the HLLD vectors come from the oracle
 and are used only to choose inputs that cover every branch.
 
-    python tests/golden/gen_golden.py [3d | recon]   (needs oracle/_ref, built here)
+    python tests/golden/gen_golden.py [3d | recon | snapshots | solvers]   (needs oracle/_ref, built here)
 """
 from __future__ import annotations
 
@@ -278,10 +278,69 @@ def recon():
          res_y=y, res_dx=dx, **out)
 
 
+def solvers():
+    """Flux types 0 (LLF), 1 (HLL), 2 (HLLC), 4 (LHLLD) of the reference's
+    flux_solver on the HLLD fixture's interface pairs, dir 1/2 (2D form) and 3
+    (3D form)."""
+    d0 = np.load(OUT / "hlld.npz", allow_pickle=False)
+    ref2, ref3 = Ref(2), Ref(3)
+    out = {}
+    for typ in (0, 1, 2, 4):
+        for d in (1, 2, 3):
+            VL = np.asfortranarray(d0[f"VL_{d}"])
+            VR = np.asfortranarray(d0[f"VR_{d}"])
+            F = farr(*VL.shape)
+            (ref2 if d < 3 else ref3).flux_solver(F, VL, VR, float(d0["ch"]), d, typ)
+            out[f"F_{typ}_{d}"] = F
+    # whole runs of each type: periodic (MC) and KH walls (minmod), 8 steps
+    rng = np.random.default_rng(66)
+    for typ in (0, 1, 2, 4):
+        for bc, lm in (("periodic", 2), ("kh", 1)):
+            U = smooth_state(rng, 22, 20)
+            if bc == "kh":
+                ref2.bc_for_U(U)
+            else:
+                ref2.bc(U)
+            U0 = U.copy(order="F")
+            dts, chs = ref_run_2d(ref2, U, 8, 0.05, 0.4, lm, bc=bc, flux_type=typ)
+            out[f"run_{typ}_{bc}_U0"], out[f"run_{typ}_{bc}_U"] = U0, U
+            out[f"run_{typ}_{bc}_dt"], out[f"run_{typ}_{bc}_ch"] = dts, chs
+    save("solvers", **out)
+
+
+def snapshots():
+    """Snapshot files written by the reference's own fileio_output (2D and 3D
+    forms), committed byte for byte with the arrays they hold."""
+    import ctypes as C
+
+    from omhd_testlib import REF_DIR, ptr  # noqa: E402
+
+    rng = np.random.default_rng(55)
+    for dim, shp in ((2, (8, 6)), (3, (6, 5, 4))):
+        L = C.CDLL(str(REF_DIR / f"libreffio{dim}d.so"))
+        U = farr(*shp, 9)
+        V = farr(*shp, 4)
+        U[...] = rng.normal(size=U.shape)
+        V[...] = rng.normal(size=V.shape)
+        axes = [np.ascontiguousarray(rng.normal(size=n)) for n in shp]
+        t = C.c_double(1.2345678901234567)
+        name = OUT / f"snapshot_ref{dim}d.dat"
+        fn = str(name).encode().ljust(256, b" ")  # character*256, blank-padded
+        ext = [C.byref(C.c_int(n)) for n in shp]
+        args = [C.c_char_p(fn)] + ext + [C.byref(t)] + [ptr(a) for a in axes] + [ptr(U), ptr(V), C.c_long(256)]
+        L.fileio_output_(*args)
+        save(f"snapshot_ref{dim}d", U=U, V=V, t=t.value, **{f"axis{i}": a for i, a in enumerate(axes)})
+        print("wrote", name, name.stat().st_size, "bytes")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "3d":
         only_3d()
     elif len(sys.argv) > 1 and sys.argv[1] == "recon":
         recon()
+    elif len(sys.argv) > 1 and sys.argv[1] == "snapshots":
+        snapshots()
+    elif len(sys.argv) > 1 and sys.argv[1] == "solvers":
+        solvers()
     else:
         main()

@@ -169,3 +169,16 @@ def test_dropin_whole_steps(api, oracle, kh):
     dts = _dropin_run(api, U, 5, 0.07, 0.4, 2, kh=kh)
     rc, dts2, _ = oracle.run(p, U2, 5)
     assert rc == 0 and bits_equal(dts, dts2) and bits_equal(U, U2)
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+@pytest.mark.parametrize("dir", (1, 2, 3))
+def test_other_flux_types_bitwise(api, typ, dir):
+    """flux_solver types 0 (LLF), 1 (HLL), 2 (HLLC), 4 (LHLLD) on the GPU,
+    bitwise to the reference's vectors."""
+    d0, d = load("hlld"), load("solvers")
+    VL, VR = np.asfortranarray(d0[f"VL_{dir}"]), np.asfortranarray(d0[f"VR_{dir}"])
+    shp = VL.shape[:-1] + (1,) * (4 - VL.ndim)
+    F = farr(*VL.shape)
+    api.flux_solver(F, VL, VR, float(d0["ch"]), shp[0], shp[1], dir, typ, kx=shp[2])
+    assert bits_equal(F, d[f"F_{typ}_{dir}"])

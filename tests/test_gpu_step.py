@@ -144,3 +144,53 @@ def test_reconnection_boundary_tables(oracle, rows):
     rc, dto, cho = oracle.run(p, Uo, 6)
     assert rc == 0
     assert bits_equal(dts, dto) and bits_equal(chs, cho) and bits_equal(Ug, Uo)
+
+
+def test_restart_from_snapshot_equals_straight_run(tmp_path):
+    """Run 7 steps, write the state in the reference's snapshot format, read
+    it into a new solver and run 5 more: the dt sequence and the final state
+    equal a straight 12-step run bitwise (the reference restarts through the
+    same files, 2D_KH_gpu/main.cuf:82-105)."""
+    from openmhd_amd import Solver, problems, snapshot
+
+    U0, dx = problems.kelvin_helmholtz(96, 64, seed=3)
+    kw = dict(cfl=0.4, lm_type=1, bc=("periodic", "wall"))
+    a = Solver(96, 64, dx, **kw)
+    a.upload(U0)
+    dts_a, _ = a.step(12, trace=True)
+    Ua = a.download()
+    a.close()
+    b = Solver(96, 64, dx, **kw)
+    b.upload(U0)
+    dts_b1, _ = b.step(7, trace=True)
+    Ub, Vb = b.download(with_v=True)
+    f = tmp_path / "field-00007.dat"
+    snapshot.write(f, b.t, snapshot.grid(96, dx, 0.0), snapshot.grid(64, dx, -32 * dx), Ub, V=Vb)
+    b.close()
+    t, _, _, Ur, _ = snapshot.read(f)
+    c = Solver(96, 64, dx, **kw)
+    c.upload(Ur)
+    dts_c, _ = c.step(5, trace=True)
+    Uc = c.download()
+    c.close()
+    assert bits_equal(np.concatenate([dts_b1, dts_c]), dts_a)
+    assert bits_equal(Uc, Ua)
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+@pytest.mark.parametrize("bc", ("periodic", "kh"))
+def test_other_flux_types_resident(typ, bc):
+    """The fused 2D step with LLF, HLL, HLLC or LHLLD: 8 steps, dt sequence and
+    state bitwise to the reference's run (periodic MC; KH walls minmod)."""
+    from openmhd_amd import Solver
+
+    d = load("solvers")
+    U0 = np.asfortranarray(d[f"run_{typ}_{bc}_U0"])
+    s = Solver(20, 18, 0.05, cfl=0.4, lm_type=2 if bc == "periodic" else 1, flux_type=typ,
+               bc=("periodic", "wall" if bc == "kh" else "periodic"), rows_per_wave=7)
+    s.upload(U0)
+    dts, chs = s.step(8, trace=True)
+    Ug = s.download()
+    s.close()
+    assert bits_equal(dts, d[f"run_{typ}_{bc}_dt"]) and bits_equal(chs, d[f"run_{typ}_{bc}_ch"])
+    assert bits_equal(Ug, d[f"run_{typ}_{bc}_U"])

@@ -245,3 +245,28 @@ def test_resistive_flux_against_the_2d_reference_form(oracle):
         got, ref = F3[:, :, 1, :] - F0, d[f"F_{dr}"] - F0
         assert np.abs(ref).max() > 1e-6  # the flux is not trivial here
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+@pytest.mark.parametrize("dir", (1, 2, 3))
+def test_other_flux_types(oracle, typ, dir):
+    """LLF, HLL, HLLC and LHLLD (flux_solver.f90 types 0, 1, 2, 4) on the HLLD
+    fixture's interface pairs, bitwise to the reference."""
+    d0, d = load("hlld"), load("solvers")
+    VL, VR = np.asfortranarray(d0[f"VL_{dir}"]), np.asfortranarray(d0[f"VR_{dir}"])
+    F = farr(*VL.shape)
+    assert oracle.flux_solver(F, VL, VR, float(d0["ch"]), dir, typ) == 0
+    assert bits_equal(F, d[f"F_{typ}_{dir}"])
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+@pytest.mark.parametrize("bc", ("periodic", "kh"))
+def test_other_flux_types_runs(oracle, typ, bc):
+    d = load("solvers")
+    U = np.asfortranarray(d[f"run_{typ}_{bc}_U0"]).copy(order="F")
+    p = make_problem(22, 20, dx=0.05, cfl=0.4, lm_type=2 if bc == "periodic" else 1, flux_type=typ,
+                     bc=("periodic", "wall" if bc == "kh" else "periodic", "periodic"))
+    rc, dts, chs = oracle.run(p, U, 8)
+    assert rc == 0
+    assert bits_equal(dts, d[f"run_{typ}_{bc}_dt"]) and bits_equal(chs, d[f"run_{typ}_{bc}_ch"])
+    assert bits_equal(U, d[f"run_{typ}_{bc}_U"])
