@@ -33,6 +33,27 @@ def prim_to_cons_fortran(ro, vx, vy, vz, p, bx, by, bz, ps):
     return U
 
 
+def brio_wu(n: int = 300):
+    """The Brio & Wu (1988) shock tube of 1D_basic/model.f90 (the active case),
+    on [-0.5, 0.5] with n cells: rho, p, By = (1, 1, 1) for x < 0 and
+    (0.125, 0.1, -1) otherwise, Bx = 0.75, v = Bz = psi = 0.  The grid is
+    built the way model.f90:16-21 builds it (x(1) = -0.5 - dx/2, then
+    x(i) = x(i-1) + dx).  Returns U(n+2, 1, 9) with its ghosts, dx and x."""
+    ix = n + 2
+    dx = (0.5 - (-0.5)) / float(ix - 2)
+    x = np.empty(ix)
+    x[0] = -0.5 - dx / 2
+    for i in range(1, ix):
+        x[i] = x[i - 1] + dx
+    left = (x < 0.0)[:, None]
+    z = np.zeros((ix, 1))
+    ro = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    by = np.where(left, 1.0, -1.0)
+    U = prim_to_cons_fortran(ro, z, z, z, p, z + 0.75, by, z, z)
+    return U, dx, x
+
+
 def orszag_tang(n: int):
     """Orszag-Tang vortex on [0,2pi]^2 with n x n cells, as 2D_basic/model.f90:
     rho = gamma^2, p = gamma, v = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0),

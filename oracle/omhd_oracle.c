@@ -943,3 +943,77 @@ int or_run(const or_problem *p, double *U, int nsteps, double *dt_trace, double 
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * The 1D shock-tube loop of 1D_basic/main.f90:56-149, on U(ix, 1, 9):
+ * zero-gradient ghosts after every stage (:42-43, :112-113, :142-143), the
+ * CFL step of set_dt (:76), nine limiter calls in x and one flux_solver call
+ * with ch = 0 (:85-98, :117-128), and the RK stages written out in the main
+ * program: U* = U + dtx (F(i-1)-F(i)) then U = 0.5 (U + U* + dtx (..))
+ * (time_type 0, :100-104, :130-134), or the midpoint form with 0.5 dtx and
+ * U + dtx (..) (time_type 1, :105-109, :135-139).  Stops when t >= tend or
+ * after max_steps steps (the reference stops at loop_max - 1 steps).
+ */
+static void outflow1d(double *A, int ix)
+{
+  for (int m = 0; m < 9; m++) {
+    A[(size_t)m * ix + (ix - 1)] = A[(size_t)m * ix + (ix - 2)];
+    A[(size_t)m * ix] = A[(size_t)m * ix + 1];
+  }
+}
+
+static int sweep1d(const double *Us, const double *V, double *VL, double *VR, double *F, int ix, int lm, int ft)
+{
+  int rc;
+  for (int m = 0; m < 4; m++)
+    if ((rc = or_limiter(V + (size_t)m * ix, VL + (size_t)m * ix, VR + (size_t)m * ix, ix, 1, 1, 1, lm))) return rc;
+  for (int m = 4; m < 9; m++)
+    if ((rc = or_limiter(Us + (size_t)m * ix, VL + (size_t)m * ix, VR + (size_t)m * ix, ix, 1, 1, 1, lm))) return rc;
+  return or_flux_solver(F, VL, VR, 0.0, ix, 1, 1, 1, ft, NULL);
+}
+
+int or_run1d(double *U, int ix, double dx, double cfl, int lm_type, int flux_type, int time_type, double tend,
+             int max_steps, double *t_out, int *nsteps_out, double *dt_trace, long *bad)
+{
+  const size_t n = (size_t)ix;
+  double *U1 = calloc(9 * n, sizeof(double)), *V = calloc(4 * n, sizeof(double));
+  double *VL = calloc(9 * n, sizeof(double)), *VR = calloc(9 * n, sizeof(double)), *F = calloc(9 * n, sizeof(double));
+  double t = 0.0, dt = 0.0, vmax = 0.0;
+  int steps = 0, rc = 0;
+  if (!U1 || !V || !VL || !VR || !F) { rc = -1; goto done; }
+  for (;;) {
+    if ((rc = or_u2v(U, V, ix, 1, 1, bad))) break;
+    if (t >= tend || steps >= max_steps) break;
+    if ((rc = or_set_dt(U, V, &vmax, &dt, dx, cfl, ix, 1, 1, 2, bad))) break;
+    const double dtx = dt / dx;
+    if ((rc = sweep1d(U, V, VL, VR, F, ix, lm_type, flux_type))) break;
+    for (int m = 0; m < 9; m++)
+      for (int i = 1; i < ix - 1; i++) {
+        const size_t q = (size_t)m * n + i;
+        if (time_type == 0)
+          U1[q] = U[q] + dtx * (F[q - 1] - F[q]);
+        else
+          U1[q] = U[q] + 0.5 * dtx * (F[q - 1] - F[q]);
+      }
+    outflow1d(U1, ix);
+    if ((rc = or_u2v(U1, V, ix, 1, 1, bad))) break;
+    if ((rc = sweep1d(U1, V, VL, VR, F, ix, lm_type, flux_type))) break;
+    for (int m = 0; m < 9; m++)
+      for (int i = 1; i < ix - 1; i++) {
+        const size_t q = (size_t)m * n + i;
+        if (time_type == 0)
+          U[q] = 0.5 * (U[q] + U1[q] + dtx * (F[q - 1] - F[q]));
+        else
+          U[q] = U[q] + dtx * (F[q - 1] - F[q]);
+      }
+    outflow1d(U, ix);
+    t = t + dt;
+    if (dt_trace) dt_trace[steps] = dt;
+    steps++;
+  }
+done:
+  free(U1); free(V); free(VL); free(VR); free(F);
+  *t_out = t;
+  *nsteps_out = steps;
+  return rc;
+}

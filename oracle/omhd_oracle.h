@@ -66,6 +66,9 @@ double or_eta_faces(const double *x, const double *y, double dx, int ix, int jx,
 double or_set_dt_resistive(double dt, double dx, double cfl, double eta_max);
 int or_step(const or_problem *p, double *U, double *dt, double *ch, long *bad);
 int or_run(const or_problem *p, double *U, int nsteps, double *dt_trace, double *ch_trace, long *bad);
+/* 1D shock tube (1D_basic/main.f90): U(ix,1,9), outflow ghosts, ch = 0 */
+int or_run1d(double *U, int ix, double dx, double cfl, int lm_type, int flux_type, int time_type, double tend,
+             int max_steps, double *t_out, int *nsteps_out, double *dt_trace, long *bad);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,8 @@ class Oracle:
         L.or_bc_face.restype = None
         L.or_step.argtypes = [C.POINTER(OrProblem), DP, DP, DP, LP]
         L.or_run.argtypes = [C.POINTER(OrProblem), DP, C.c_int, DP, DP, LP]
+        L.or_run1d.argtypes = [DP, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                               DP, IP, DP, LP]
 
     @staticmethod
     def _dims(a, nvar_axis=True):
@@ -156,6 +158,14 @@ class Oracle:
         emax = self.L.or_eta_faces(ptr(np.ascontiguousarray(x, dtype=np.float64)),
                                    ptr(np.ascontiguousarray(y, dtype=np.float64)), dx, ix, jx, eta0, eta1, w, ptr(out))
         return out, emax
+
+    def run1d(self, U, dx, cfl=0.8, lm=1, flux=3, tt=0, tend=0.2, max_steps=29999):
+        """1D shock-tube loop on U(ix,1,9) in place: (rc, t, dt trace)."""
+        dts = np.zeros(max_steps)
+        t, ns, bad = C.c_double(), C.c_int(), C.c_long(-1)
+        rc = self.L.or_run1d(ptr(U), U.shape[0], dx, cfl, lm, flux, tt, tend, max_steps, C.byref(t), C.byref(ns),
+                             ptr(dts), C.byref(bad))
+        return rc, t.value, dts[:ns.value]
 
     def run(self, prob, U, nsteps):
         dts = np.zeros(nsteps)

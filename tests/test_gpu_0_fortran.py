@@ -1,6 +1,7 @@
-"""The Fortran side of the boundary: the project's reference-style driver
-(openmhd_amd/fortran/main2d.f90) linked against the libomhd shims, and the
-resident-step driver, both against the oracle on the same snapshot.  Bitwise."""
+"""The Fortran side of the boundary: the project's reference-style drivers
+(openmhd_amd/fortran/main2d.f90, main1d.f90) linked against the libomhd shims,
+and the resident-step driver, all against the oracle on the same initial
+state.  Bitwise."""
 import subprocess
 from pathlib import Path
 
@@ -57,3 +58,28 @@ def test_driver_matches_oracle(tmp_path, oracle, exe, kh):
     assert rc == 0
     assert bits_equal(dts, dto)
     assert bits_equal(Ug, Uo)
+
+
+# the 1D shock tube (1D_basic/main.f90) through the drop-in shims: every flux
+# type, every limiter, both integrators, to t = 0.2 (about 300 steps)
+@pytest.mark.parametrize("flux,lm,tt", [(3, 1, 0), (3, 2, 1), (3, 0, 0), (0, 1, 0), (1, 3, 0), (2, 4, 1), (4, 2, 0)])
+def test_driver_1d_shock_tube_matches_oracle(tmp_path, oracle, flux, lm, tt):
+    from openmhd_amd import problems, snapshot
+
+    out, tr = tmp_path / "out.dat", tmp_path / "trace.bin"
+    cmd = [str(FDIR / "main1d_gpu"), "--flux", str(flux), "--lm", str(lm), "--tt", str(tt), "--output", str(out),
+           "--trace", str(tr)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "steps" in r.stdout, r.stdout + r.stderr
+    t, _, _, Ug, Vg = snapshot.read(out)
+    dts = np.fromfile(tr)
+    Uo, dx, _ = problems.brio_wu(300)
+    rc, to, dto = oracle.run1d(Uo, dx, 0.8, lm, flux, tt, 0.2)
+    assert rc == 0
+    assert len(dts) == len(dto) > 100
+    assert bits_equal(dts, dto)
+    assert t == to
+    assert bits_equal(Ug, Uo)
+    Vo = np.zeros((Uo.shape[0], 1, 4), order="F")
+    assert oracle.u2v(Uo, Vo)[0] == 0
+    assert bits_equal(Vg, Vo)
