@@ -247,7 +247,7 @@ def bench_adjoint(a):
     from openmhd_amd import Adjoint, Rollout, Solver, problems
 
     torch.cuda.set_device(0)
-    n, nsteps, every = a.n if a.n != 4096 else 512, a.steps if a.steps != 500 else 200, 20
+    n, nsteps, every = a.n if a.n != 4096 else 512, a.steps, 20
     dx = 2 * np.pi / n
     s = Solver(n, n, dx, cfl=0.4, lm_type=2)
     modes = problems.fourier_modes(1)
@@ -333,7 +333,9 @@ def bench_adjoint(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the configuration's own count: KH 500, ensemble 300, adjoint 200, "
+                         "3D 100 including the warmup)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
@@ -352,6 +354,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = {"kh": 500, "ensemble": 300, "adjoint": 200, "3d": max(1, 100 - a.warmup)}[a.workload]
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on one GPU: the grid is one domain on one MI355X and sharding it "
                  "across GPUs is not built -- run with --gpus 1")

@@ -245,6 +245,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     // stage 2: issue the loads of U^n for the row combined at the end of this
     // iteration now, so their latency hides behind the two flux evaluations
     double a0p[NV], usp[NV];
+    // TVD stage 2 folds U^n + U* (the first sum of rk_tvd22) before the
+    // march-axis flux, so the U* loads go out as one batch
+    constexpr bool FOLD = FINAL && STAGE == 2 && TT == 0 && PREF;
     if constexpr (FINAL && STAGE == 2 && PREF) {
       if (k >= 4 && writer) {
         const long long o = fixoff + (long long)(rn - 2) * rstride + c;
@@ -308,6 +311,16 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         limit<LIM>(lds.ring[sa][m][lane + 1], lds.ring[sb][m][lane + 1], lds.ring[sc][m][lane + 1], Rf[m], wLyB);
         Lf[m] = lds.wly[m][lane];  // left state of the interface above row rn-2
         lds.wly[m][lane] = wLyB;
+      }
+      if constexpr (FOLD) {
+        if (k >= 4 && writer) {
+          const long long o = fixoff + (long long)(rn - 2) * rstride + c;
+          double usf[NV];
+#pragma unroll
+          for (int m = 0; m < NV; m++) usf[m] = p.us[m * plane + o];
+#pragma unroll
+          for (int m = 0; m < NV; m++) a0p[m] = (m == 8 ? glm_f * a0p[m] : a0p[m]) + usf[m];
+        }
       }
       if (k >= 3) {
         // march-axis interface between rows rn-2 and rn-1
@@ -375,12 +388,16 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             } else {
 #pragma unroll
               for (int m = 0; m < NV; m++) {
-                double a0;
-                if constexpr (PREF) a0 = a0p[m];
-                else a0 = p.u0[m * plane + o];
-                if (m == 8) a0 = glm_f * a0;
-                if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + dtx * d[m]);  // rk_tvd22
-                else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
+                if constexpr (FOLD) {
+                  o9[m] = 0.5 * (a0p[m] + dtx * d[m]);  // rk_tvd22: a0p holds U^n + U*
+                } else {
+                  double a0;
+                  if constexpr (PREF) a0 = a0p[m];
+                  else a0 = p.u0[m * plane + o];
+                  if (m == 8) a0 = glm_f * a0;
+                  if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + dtx * d[m]);  // rk_tvd22
+                  else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
+                }
               }
               o9[8] = glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
             }
