@@ -109,7 +109,7 @@ typedef struct omhd_config {
   double dx;            /* cell size, dx == dy (rk.f90:4-5)                 */
   double cfl;           /* CFL number (main.f90:15)                         */
   int32_t lm_type;      /* 0 flat, 1 minmod, 2 MC, 3 van Leer, 4 Koren      */
-  int32_t flux_type;    /* 0 LLF, 1 HLL, 2 HLLC, 3 HLLD, 4 LHLLD (0-2, 4: 2D) */
+  int32_t flux_type;    /* 0 LLF, 1 HLL, 2 HLLC, 3 HLLD, 4 LHLLD (0-2, 4: ideal MHD only) */
   int32_t time_type;    /* 0 TVD-RK2, 1 RK2 (main.f90:20-21)                */
   int32_t bc_type[6];   /* OMHD_BC_* per face: x-, x+, y-, y+, z-, z+       */
   double bc_sign[54];   /* mirror sign per face and slot (6 x 9)            */

@@ -431,13 +431,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   march_tile<LIM, STAGE, TT, PASS, RES, (LIM != 0)>(p, t, lds);
 }
 
-// 2D step with one of the other flux types (p.ftype), same structure
-template <int LIM, int STAGE, int TT>
+// Ideal-MHD step (2D, or the 3D XY and Z passes) with one of the other flux
+// types (p.ftype), same structure
+template <int LIM, int STAGE, int TT, int PASS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march_gen(const Sweep p) {
   __shared__ LdsOf<0> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
-  march_tile<LIM, STAGE, TT, PASS_2D, 0, false, true>(p, t, lds);
+  march_tile<LIM, STAGE, TT, PASS, 0, false, true>(p, t, lds);
 }
 
 // Ensemble: one persistent launch per RK stage for every member.  The tile
@@ -583,21 +584,29 @@ march_fn pick2(int lim, int stage, int tt) {
   }
 }
 
-template <int LIM>
+template <int LIM, int PASS>
 march_fn pick_gen(int stage, int tt) {
-  if (stage == 1) return tt == 0 ? k_march_gen<LIM, 1, 0> : k_march_gen<LIM, 1, 1>;
-  return tt == 0 ? k_march_gen<LIM, 2, 0> : k_march_gen<LIM, 2, 1>;
+  if (stage == 1) return tt == 0 ? k_march_gen<LIM, 1, 0, PASS> : k_march_gen<LIM, 1, 1, PASS>;
+  return tt == 0 ? k_march_gen<LIM, 2, 0, PASS> : k_march_gen<LIM, 2, 1, PASS>;
 }
 
+template <int PASS>
+march_fn pick_gen2(int lim, int stage, int tt) {
+  switch (lim) {
+    case 0: return pick_gen<0, PASS>(stage, tt);
+    case 1: return pick_gen<1, PASS>(stage, tt);
+    case 2: return pick_gen<2, PASS>(stage, tt);
+    case 3: return pick_gen<3, PASS>(stage, tt);
+    default: return pick_gen<4, PASS>(stage, tt);
+  }
+}
+
+// gen: a flux type other than HLLD (ideal MHD only; omhd_create enforces it)
 march_fn march_kernel(int pass, int res, int lim, int stage, int tt, bool gen = false) {
-  if (pass == PASS_2D && gen) {
-    switch (lim) {
-      case 0: return pick_gen<0>(stage, tt);
-      case 1: return pick_gen<1>(stage, tt);
-      case 2: return pick_gen<2>(stage, tt);
-      case 3: return pick_gen<3>(stage, tt);
-      default: return pick_gen<4>(stage, tt);
-    }
+  if (gen) {
+    if (pass == PASS_2D) return pick_gen2<PASS_2D>(lim, stage, tt);
+    if (pass == PASS_XY) return pick_gen2<PASS_XY>(lim, stage, tt);
+    return pick_gen2<PASS_Z>(lim, stage, tt);
   }
   if (pass == PASS_2D) return pick2<PASS_2D, 0>(lim, stage, tt);
   if (pass == PASS_XY) return res ? pick2<PASS_XY, 1>(lim, stage, tt) : pick2<PASS_XY, 0>(lim, stage, tt);
@@ -658,8 +667,9 @@ extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
     return set_error(OMHD_ERR_EXTENT, "need nx, ny (, nz) >= 2", cfg->nx, cfg->ny, nz);
   if (cfg->lm_type < 0 || cfg->lm_type > 4) return set_error(OMHD_ERR_TYPE, "unknown limiter", cfg->lm_type);
   if (cfg->flux_type < 0 || cfg->flux_type > 4) return set_error(OMHD_ERR_TYPE, "unknown flux_type", cfg->flux_type);
-  if (cfg->flux_type != 3 && (cfg->ndim != 2 || cfg->resistive))
-    return set_error(OMHD_ERR_TYPE, "flux types other than HLLD are built for the 2D step", cfg->flux_type);
+  if (cfg->flux_type != 3 && cfg->resistive)
+    return set_error(OMHD_ERR_TYPE, "flux types other than HLLD are built for ideal MHD (no resistivity)",
+                     cfg->flux_type);
   if (cfg->time_type != 0 && cfg->time_type != 1) return set_error(OMHD_ERR_TYPE, "unknown time_type", cfg->time_type);
   if (!(cfg->dx > 0) || !(cfg->cfl > 0)) return set_error(OMHD_ERR_ARG, "dx and cfl must be positive");
   if (cfg->resistive && cfg->ndim != 3)
@@ -848,7 +858,7 @@ int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, 
   Sweep pa = base_sweep(h, three ? PASS_XY : PASS_2D);
   Sweep pz = three ? base_sweep(h, PASS_Z) : pa;
   const march_fn a = march_kernel(three ? PASS_XY : PASS_2D, res, c.lm_type, stage, c.time_type, c.flux_type != 3);
-  const march_fn z = three ? march_kernel(PASS_Z, res, c.lm_type, stage, c.time_type) : nullptr;
+  const march_fn z = three ? march_kernel(PASS_Z, res, c.lm_type, stage, c.time_type, c.flux_type != 3) : nullptr;
   const double cr = 0.2;                          // glm_ss2.f90:18
   const double glm_f = exp(-0.5 * dt * ch / cr);  // glm_ss2.f90:27, host libm
   // stage 1: U* = U + dtx dF (rk_tvd21: dt/dx, rk_std21: 0.5*dt/dx); stage 2: dt/dx

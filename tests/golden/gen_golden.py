@@ -281,7 +281,7 @@ def recon():
 def solvers():
     """Flux types 0 (LLF), 1 (HLL), 2 (HLLC), 4 (LHLLD) of the reference's
     flux_solver on the HLLD fixture's interface pairs, dir 1/2 (2D form) and 3
-    (3D form)."""
+    (3D form); whole 2D runs (periodic and KH walls) and 3D periodic runs."""
     d0 = np.load(OUT / "hlld.npz", allow_pickle=False)
     ref2, ref3 = Ref(2), Ref(3)
     out = {}
@@ -305,6 +305,16 @@ def solvers():
             dts, chs = ref_run_2d(ref2, U, 8, 0.05, 0.4, lm, bc=bc, flux_type=typ)
             out[f"run_{typ}_{bc}_U0"], out[f"run_{typ}_{bc}_U"] = U0, U
             out[f"run_{typ}_{bc}_dt"], out[f"run_{typ}_{bc}_ch"] = dts, chs
+    # 3D periodic runs of each type (3D_basic routines), MC, 4 steps
+    from omhd_testlib import ref_run_3d  # noqa: E402
+
+    for typ in (0, 1, 2, 4):
+        U = smooth_state(rng, 14, 12, 10)
+        ref3.bc(U)
+        U0 = U.copy(order="F")
+        dts, chs = ref_run_3d(ref3, U, 4, 0.1, 0.4, 2, flux_type=typ)
+        out[f"run3d_{typ}_U0"], out[f"run3d_{typ}_U"] = U0, U
+        out[f"run3d_{typ}_dt"], out[f"run3d_{typ}_ch"] = dts, chs
     save("solvers", **out)
 
 

@@ -9,11 +9,12 @@ from test_oracle_golden import bits_equal, load
 pytestmark = pytest.mark.gpu
 
 
-def run_both(oracle, U, nsteps, dx, lm=2, tt=0, bc=("periodic",) * 3, rows=0, resistivity=None, origin=(0.0, 0.0)):
+def run_both(oracle, U, nsteps, dx, lm=2, tt=0, bc=("periodic",) * 3, rows=0, resistivity=None, origin=(0.0, 0.0),
+             flux=3):
     from openmhd_amd import Solver, snapshot
 
     ix, jx, kx = U.shape[:3]
-    p = make_problem(ix, jx, kx, dx=dx, cfl=0.4, lm_type=lm, time_type=tt, bc=bc)
+    p = make_problem(ix, jx, kx, dx=dx, cfl=0.4, lm_type=lm, time_type=tt, bc=bc, flux_type=flux)
     eta = None
     if resistivity is not None:
         x = snapshot.grid(ix - 2, dx, origin[0])
@@ -23,7 +24,7 @@ def run_both(oracle, U, nsteps, dx, lm=2, tt=0, bc=("periodic",) * 3, rows=0, re
         p.eta_max = emax
     oracle.bc_u(U, p)
     s = Solver(ix - 2, jx - 2, dx, nz=kx - 2, cfl=0.4, lm_type=lm, time_type=tt, bc=bc, rows_per_wave=rows,
-               resistivity=resistivity, origin=origin)
+               resistivity=resistivity, origin=origin, flux_type=flux)
     s.upload(U)
     dts, chs = s.step(nsteps, trace=True)
     Ug = s.download()
@@ -71,4 +72,32 @@ def test_resistive_harris(oracle, bc):
     U, origin = problems.harris_sheet_3d(68, 40, 24, dx, noise=1e-2)
     dts, dto, Ug, Uo = run_both(oracle, U, 5, dx, 1, 0, bc=bc, rows=11, resistivity=(1e-3, 1e-2, 2.0),
                                 origin=origin)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+def test_other_flux_types_golden(typ):
+    """The 3D step with LLF, HLL, HLLC or LHLLD: the reference's 3D periodic run
+    (3D_basic routines, MC, 4 steps), dt sequence and state bitwise."""
+    from openmhd_amd import Solver
+
+    d = load("solvers")
+    U = np.asfortranarray(d[f"run3d_{typ}_U0"]).copy(order="F")
+    ix, jx, kx = U.shape[:3]
+    s = Solver(ix - 2, jx - 2, 0.1, nz=kx - 2, cfl=0.4, lm_type=2, flux_type=typ)
+    s.upload(U)
+    dts, chs = s.step(4, trace=True)
+    Ug = s.download()
+    s.close()
+    assert bits_equal(dts, d[f"run3d_{typ}_dt"]) and bits_equal(chs, d[f"run3d_{typ}_ch"])
+    assert bits_equal(Ug, d[f"run3d_{typ}_U"])
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+@pytest.mark.parametrize("lm,tt,bc", ((1, 0, ("periodic", "wall", "periodic")), (0, 1, ("wall", "periodic", "wall")),
+                                      (4, 0, ("periodic",) * 3)))
+def test_other_flux_types_oracle(oracle, typ, lm, tt, bc):
+    """Several strips and chunks, walls, every integrator: bitwise to the oracle."""
+    U = smooth_state(np.random.default_rng(20 + typ), 66, 26, 18)
+    dts, dto, Ug, Uo = run_both(oracle, U, 3, 0.1, lm, tt, bc=bc, rows=7, flux=typ)
     assert bits_equal(dts, dto) and bits_equal(Ug, Uo)

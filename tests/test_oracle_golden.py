@@ -270,3 +270,16 @@ def test_other_flux_types_runs(oracle, typ, bc):
     assert rc == 0
     assert bits_equal(dts, d[f"run_{typ}_{bc}_dt"]) and bits_equal(chs, d[f"run_{typ}_{bc}_ch"])
     assert bits_equal(U, d[f"run_{typ}_{bc}_U"])
+
+
+@pytest.mark.parametrize("typ", (0, 1, 2, 4))
+def test_other_flux_types_3d_runs(oracle, typ):
+    """3D periodic runs with LLF, HLL, HLLC and LHLLD (3D_basic routines), MC,
+    4 steps: bitwise to the flang build."""
+    d = load("solvers")
+    U = np.asfortranarray(d[f"run3d_{typ}_U0"]).copy(order="F")
+    ix, jx, kx = U.shape[:3]
+    rc, dts, chs = oracle.run(make_problem(ix, jx, kx, dx=0.1, cfl=0.4, lm_type=2, flux_type=typ), U, 4)
+    assert rc == 0
+    assert bits_equal(dts, d[f"run3d_{typ}_dt"]) and bits_equal(chs, d[f"run3d_{typ}_ch"])
+    assert bits_equal(U, d[f"run3d_{typ}_U"])
