@@ -246,7 +246,11 @@ size_t plane_minloc(const double *plane, size_t n, bool dev, double &val) {
   double *tmp = nullptr;
   if (dev) {
     tmp = (double *)malloc(n * sizeof(double));
-    (void)hipMemcpy(tmp, plane, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (!tmp || hipMemcpy(tmp, plane, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+      free(tmp);
+      val = 0.0;
+      return 0;  // the location is diagnostic only; the caller still reports the error code
+    }
     h = tmp;
   }
   size_t best = 0;

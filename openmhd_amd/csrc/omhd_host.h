@@ -51,21 +51,29 @@ inline bool is_device_ptr(const void *p) {
 
 // A device view of a caller array: used in place when it is device memory,
 // otherwise a device copy (uploaded if `in`, downloaded at finish() if `out`).
+// A copy is freed, stream-ordered, by release() or when the view goes out of
+// scope, so an entry point that returns early on an error frees it too.
 struct Staged {
   double *dev = nullptr;
   double *host = nullptr;
   size_t n = 0;
   bool owned = false, out = false;
+  hipStream_t stream = nullptr;
+  Staged() = default;
+  Staged(const Staged &) = delete;
+  Staged &operator=(const Staged &) = delete;
+  ~Staged() { release(stream); }
   int init(const double *p, size_t count, bool in, bool out_, hipStream_t s) {
     n = count;
     out = out_;
+    stream = s;
     if (is_device_ptr(p)) {
       dev = const_cast<double *>(p);
       return OMHD_OK;
     }
     host = const_cast<double *>(p);
-    owned = true;
     OMHD_HIP(hipMallocAsync((void **)&dev, n * sizeof(double), s));
+    owned = true;
     if (in) OMHD_HIP(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, s));
     return OMHD_OK;
   }
@@ -76,6 +84,7 @@ struct Staged {
   void release(hipStream_t s) {
     if (owned && dev) (void)hipFreeAsync(dev, s);
     dev = nullptr;
+    owned = false;
   }
 };
 
