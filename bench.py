@@ -153,7 +153,7 @@ def bench_3d(a):
     eta0, eta1, w = 1e-3, 1e-2, 2.0
     U, origin = problems.harris_sheet_3d(nx, ny, nz, dx)
     s = Solver(nx, ny, dx, nz=nz, cfl=0.4, lm_type=1, bc=("periodic", "wall", "periodic"),
-               resistivity=None if a.no_res else (eta0, eta1, w), origin=origin)
+               resistivity=None if a.no_res else (eta0, eta1, w), origin=origin, rows_per_wave=a.rows)
     s.upload(U)
     W0 = s.get_state()
     del U
