@@ -613,6 +613,30 @@ march_fn march_kernel(int pass, int res, int lim, int stage, int tt, bool gen = 
   return res ? pick2<PASS_Z, 1>(lim, stage, tt) : pick2<PASS_Z, 0>(lim, stage, tt);
 }
 
+// Chunk height when the caller leaves it to the library.  Each wavefront
+// marches its chunk row after row, so once a grid has fewer than a few rounds
+// of wavefronts (slots = CUs x 2 per SIMD x 4 SIMDs) the step time follows the
+// chunk height, not the cell count: KH 512^2 ran 2.5x faster with 8-row chunks
+// than with 28.  Such grids take the height that minimises
+// ceil(tiles / slots) * (rows + 3), the 3 being the rows loaded before a chunk's
+// first update (ties to the taller chunk).  Larger grids keep the measured
+// heights: 28 rows in 2D (KH 4096^2), 24 in 3D (Harris 512x256x256).
+int default_rows(int nx, int ny, int nz, int nd, long long slots) {
+  const long long strips = (nx + STRIP - 1) / STRIP, slices = nd == 3 ? nz : 1;
+  auto tiles = [&](int r) { return strips * slices * ((ny + r - 1) / r); };
+  if (tiles(24) >= 4 * slots) return nd == 2 ? 28 : 24;
+  int best = 28;
+  long long best_cost = -1;
+  for (int r : {28, 24, 20, 16, 12, 8}) {
+    const long long cost = (tiles(r) + slots - 1) / slots * (r + 3);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = r;
+    }
+  }
+  return best;
+}
+
 unsigned sign_mask(const double *s9) {
   unsigned m = 0;
   for (int i = 0; i < NV; i++)
@@ -685,10 +709,19 @@ extern "C" int omhd_create(const omhd_config *cfg, omhd_handle **out) {
   h->pitch = (cfg->nx + 7) & ~7;  // 64-byte aligned rows
   h->zstride = (long long)h->pitch * cfg->ny;
   h->plane = ((h->zstride * nz) + 63) & ~63LL;
-  // default chunk heights, measured: on KH 4096^2, chunks of 16, 24 or 32 rows
-  // ran about 2% slower than 20 or 28 (28 best); on the 512x256x256 Harris
-  // sheet 24 was best, with 20 and 28 within 0.5%
-  h->rows = cfg->rows_per_wave > 0 ? cfg->rows_per_wave : (cfg->ndim == 2 ? 28 : 24);
+  // default chunk heights (default_rows), measured: on KH 4096^2, chunks of 16,
+  // 24 or 32 rows ran about 2% slower than 20 or 28 (28 best); on the
+  // 512x256x256 Harris sheet 24 was best, with 20 and 28 within 0.5%
+  if (cfg->rows_per_wave > 0) {
+    h->rows = cfg->rows_per_wave;
+  } else {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+    h->rows = default_rows(cfg->nx, cfg->ny, nz, cfg->ndim, 8LL * cus);
+  }
   for (int a = 0; a < 3; a++) {
     h->faces.bc[a] = cfg->bc_type[2 * a];
     h->faces.slo[a] = sign_mask(cfg->bc_sign + 18 * a);
@@ -1395,7 +1428,7 @@ extern "C" int omhd_batch_autotune(omhd_batch *b, int32_t trial_steps, int32_t *
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const int cand_rows[] = {12, 16, 24, 32, 48};
+  const int cand_rows[] = {8, 12, 16, 20, 24, 28, 32, 48};
   const int cand_grid[] = {0, 8 * cus, 16 * cus};  // one group per tile; persistent at 2 or 4 groups per SIMD
   const std::vector<double> vmax0 = b->vmax, t0 = b->t;
   const bool dmode = b->device_dt, gmode = b->use_graph;
