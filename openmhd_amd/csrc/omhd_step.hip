@@ -262,6 +262,17 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         for (int m = 0; m < NV; m++) usp[m] = p.us[m * plane + o];
       }
     }
+    // Z pass, stage 1, ideal MHD: the XY pass's partial flux difference for
+    // the same row (elsewhere the registers for it spill)
+    constexpr bool ZPRE = PASS == PASS_Z && PREF && RES == 0 && STAGE == 1;
+    double azp[NV];
+    if constexpr (ZPRE) {
+      if (k >= 4 && writer) {
+        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
+#pragma unroll
+        for (int m = 0; m < NV; m++) azp[m] = p.acc[m * plane + o];
+      }
+    }
 
     // x-sweep of row r = rn-2: tq = (F(i-1) - F(i)) + G(j-1)   (rk.f90:30-31 order)
     double tq[NV];
@@ -372,7 +383,12 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             for (int m = 0; m < NV; m++) p.acc[m * plane + o] = tq[m] - G[m];  // ((F-F)+G)-G
           } else {
 #pragma unroll
-            for (int m = 0; m < NV; m++) d[m] = p.acc[m * plane + o] + Gl[m] - G[m];  // (.. + H(k-1)) - H(k)
+            for (int m = 0; m < NV; m++) {
+              double a;
+              if constexpr (ZPRE) a = azp[m];
+              else a = p.acc[m * plane + o];
+              d[m] = a + Gl[m] - G[m];  // (.. + H(k-1)) - H(k)
+            }
           }
           if constexpr (FINAL) {
             double o9[NV];
