@@ -19,14 +19,20 @@
  * Parity status per function (what pins it):
  *   or_u2v            flang vectors + closed form (uniform state)
  *   or_limiter        flang vectors incl. -0.0 and sign-mismatch cases
- *   or_flux_solver    flang vectors for every HLLD branch x dir 1/2/3, and
- *                     for the LLF, HLL, HLLC and LHLLD types x dir 1/2/3
+ *   or_flux_solver,   flang vectors for every HLLD branch x dir 1/2/3, and
+ *   or_hlld, or_riemann  for the LLF, HLL, HLLC and LHLLD types x dir 1/2/3
  *   or_glm_ss2        flang vectors (single multiply)
  *   or_rk_*           flang vectors
  *   or_set_dt         flang vectors
  *   or_bc_*           flang vectors (periodic, KH walls)
  *   or_step / or_run  OT 200^2 MC 1000-step dt trace and final-state hash from
- *                     the flang build; KH walls 100 steps; 3D periodic 4 steps
+ *                     the flang build; KH walls 100 steps; 3D periodic 4 steps;
+ *                     every flux type over 2D (periodic, KH walls) and 3D runs
+ *   or_run1d          the flang build of 1D_basic's routines under the same
+ *                     driver loop, bitwise for 8 (flux, limiter, integrator)
+ *                     cases; known answers of the Brio-Wu tube (conservation
+ *                     at the outflow faces, frozen Bx/psi/vz/Bz, mirror
+ *                     symmetry of a symmetric tube)
  *   or_eta_faces,     closed form of the resistivity profile; the resistive dt
  *   or_set_dt_resistive  limit 0.5 cfl dx^2 / eta_max (flux_resistive.cuf:2-9)
  *   or_flux_resistive the reference's 3D form is CUDA Fortran only; pinned by
