@@ -597,7 +597,8 @@ extern "C" int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, doubl
   OMHD_HIP(hipStreamSynchronize(h->stream));
   int err[4];
   memcpy(err, &h->red_host[1], sizeof(err));
-  if (err[0]) return set_error(err[0], "invalid state while replaying the forward pass", err[1], err[2], err[3]);
+  if (err[0])  // the low byte is the OMHD_ERR_* code; a higher bit marks the stage-2 state
+    return set_error(err[0] & 0xff, "invalid state while replaying the forward pass", err[1], err[2], err[3]);
   if (a->geta) {
     a->geta_host.resize(a->neta());
     OMHD_HIP(hipMemcpy(a->geta_host.data(), a->geta, a->neta() * sizeof(double), hipMemcpyDeviceToHost));

@@ -58,7 +58,8 @@ struct Sweep {
   double dtx, glm_f, ch, ch2, dx;
   const double *eta;          // resistive: 3 face arrays (ix*jx each) or null
   unsigned long long *vmax_bits;  // stage 2: max speed of U^{n+1}
-  int *err;                       // [code, i, j, k] of the first failure
+  int *err;                       // [code, i, j, k] of the first failure in a state read
+  int *err_next;                  // stage 2, in U^{n+1}: [positivity code, non-finite speed]
   const double *dpar;  // device-side step parameters (DevPar) or null: use the fields above
   double *U, *U1;      // ensemble members: their two state buffers
   int ftype;           // flux type of the generic-solver kernels (0 LLF, 1 HLL, 2 HLLC, 4 LHLLD)
@@ -94,6 +95,10 @@ __device__ __forceinline__ double flip(double v, unsigned mask, int m) {
   const long long sbit = (long long)((mask >> m) & 1u) << 63;
   return __longlong_as_double(__double_as_longlong(v) ^ sbit);
 }
+
+// Failures found while a stage reads its state carry ERR_IN_USTAR when that
+// state is U* (stage 2), so the host knows which buffer failed.
+constexpr int ERR_IN_USTAR = 0x200;
 
 __device__ __forceinline__ void record_error(int *err, int code, int i, int j, int k) {
   if (atomicCAS(err, 0, code) == 0) {
@@ -207,7 +212,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, dummy);
         const int jj = MD == 1 ? rm : slice;
         const int kk = MD == 1 ? (ND == 3 ? slice + 2 : 0) : rm + 2;
-        record_error(p.err, (W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE, cm + 2, jj + 2, kk);
+        record_error(p.err, ((W[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE) | (STAGE == 2 ? ERR_IN_USTAR : 0),
+                     cm + 2, jj + 2, kk);
       }
       if constexpr (RES != 0) {
         // transverse B samples of this row for the resistive face currents
@@ -419,9 +425,11 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             }
 #pragma unroll
             for (int m = 0; m < NV; m++) p.out[m * plane + o] = o9[m];
-            if constexpr (STAGE == 2) {  // next step's set_dt on U^{n+1}
+            if constexpr (STAGE == 2) {  // next step's u2v and set_dt on U^{n+1}
               double w2[NV];
               cons_to_prim(o9, w2);
+              if (!(w2[3] > 0.0) || !(o9[4] > 0.0))
+                (void)atomicCAS(p.err_next, 0, (w2[3] > 0.0) ? OMHD_ERR_DENSITY : OMHD_ERR_PRESSURE);
               const double vt = fastest_speed<ND>(o9, w2);
               vloc = (vt > vloc || !(vt == vt)) ? vt : vloc;
             }
@@ -433,7 +441,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     }
   }
   if constexpr (FINAL && STAGE == 2) {
-    if (!(vloc >= 0.0)) record_error(p.err, OMHD_ERR_DT, 0, 0, 0);
+    if (!(vloc >= 0.0)) (void)atomicCAS(p.err_next + 1, 0, OMHD_ERR_DT);
     const double vm = wave_max(vloc >= 0.0 ? vloc : 0.0);
     if (lane == 0) atomic_max_nonneg(p.vmax_bits, vm);
   }
@@ -444,6 +452,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ LdsOf<RES> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
+  if (STAGE == 2 && p.err[0] != 0) return;  // stage 1 failed: leave U^n as it is
   march_tile<LIM, STAGE, TT, PASS, RES, (LIM != 0)>(p, t, lds);
 }
 
@@ -454,6 +463,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ LdsOf<0> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
+  if (STAGE == 2 && p.err[0] != 0) return;  // stage 1 failed: leave U^n as it is
   march_tile<LIM, STAGE, TT, PASS, 0, false, true>(p, t, lds);
 }
 
@@ -665,11 +675,13 @@ unsigned sign_mask(const double *s9) {
 
 using namespace omhd;
 
-static int read_red(omhd_handle *h, double &vmax, int err[4]) {
-  OMHD_HIP(hipMemcpyAsync(h->red_host, h->red, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+// red: [max speed bits | err[4] | err_next[2]]
+static int read_red(omhd_handle *h, double &vmax, int err[4], int next[2] = nullptr) {
+  OMHD_HIP(hipMemcpyAsync(h->red_host, h->red, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
   OMHD_HIP(hipStreamSynchronize(h->stream));
   memcpy(&vmax, &h->red_host[0], sizeof(double));
   memcpy(err, &h->red_host[1], 4 * sizeof(int));
+  if (next) memcpy(next, &h->red_host[3], 2 * sizeof(int));
   return OMHD_OK;
 }
 
@@ -816,6 +828,59 @@ static int state_error(const int err[4]) {
   return set_error(err[0], what, err[1], err[2], err[3]);
 }
 
+// The reference's u2v stop (u2v.f90:34-43) on a state that failed its
+// positivity test: over the ghost-inclusive array, prmin is the minimum of
+// E(1,1,1) and every pressure, and if prmin <= 0 the report is the first
+// cell (column-major) of minimum pressure with that pressure; otherwise the
+// same for the density.  `state` is the resident buffer that failed.  If the
+// diagnostic cannot run, the kernel's own report (code and first cell found)
+// is returned instead.
+static int positivity_error(omhd_handle *h, const double *state, const int err[4]) {
+  const omhd_config &c = h->cfg;
+  const size_t n = nfortran(h);
+  double *dU = nullptr, *dV = nullptr;
+  std::vector<double> p(n), ro(n);
+  double e11 = 0.0;
+  bool ok_ = hipMalloc((void **)&dU, 9 * n * sizeof(double)) == hipSuccess &&
+             hipMalloc((void **)&dV, 4 * n * sizeof(double)) == hipSuccess;
+  if (ok_) {
+    k_to_fortran<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(state, h->plane, h->zstride, h->pitch, c.nx, c.ny,
+                                                                     c.nz, h->nd, h->faces, dU, dV);
+    ok_ = hipGetLastError() == hipSuccess &&
+          hipMemcpyAsync(p.data(), dV + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+          hipMemcpyAsync(ro.data(), dU + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+          hipMemcpyAsync(&e11, dU + 3 * n, sizeof(double), hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+          hipStreamSynchronize(h->stream) == hipSuccess;
+  }
+  if (dU) (void)hipFree(dU);
+  if (dV) (void)hipFree(dV);
+  if (!ok_) {
+    (void)hipGetLastError();
+    return state_error(err);
+  }
+  size_t ip = 0, ir = 0;
+  for (size_t q = 1; q < n; q++) {
+    if (p[q] < p[ip]) ip = q;
+    if (ro[q] < ro[ir]) ir = q;
+  }
+  const double prmin = e11 < p[ip] ? e11 : p[ip];  // u2v.f90 starts prmin at U(1,1,en)
+  const double rhomin = ro[0] < ro[ir] ? ro[0] : ro[ir];
+  const int ix = c.nx + 2, jx = c.ny + 2;
+  const size_t q = prmin <= 0 ? ip : ir;
+  const int i = (int)(q % ix) + 1, j = (int)(q / ix % jx) + 1, k = (int)(q / ((size_t)ix * jx)) + 1;
+  if (prmin <= 0) return set_error(OMHD_ERR_PRESSURE, "negative pressure", i, j, k, p[ip]);
+  if (rhomin <= 0) return set_error(OMHD_ERR_DENSITY, "negative density", i, j, k, ro[ir]);
+  return state_error(err);  // the kernel saw a non-positive value the arrays no longer show
+}
+
+// After a step: a failure in a state the stages read (U^n, or U* when tagged
+// ERR_IN_USTAR) stops the step before it completes.
+static int read_error(omhd_handle *h, const int err[4]) {
+  int e[4] = {err[0] & ~ERR_IN_USTAR, err[1], err[2], err[3]};
+  if (e[0] != OMHD_ERR_PRESSURE && e[0] != OMHD_ERR_DENSITY) return state_error(e);
+  return positivity_error(h, (err[0] & ERR_IN_USTAR) ? h->U1 : h->U, e);
+}
+
 static int cfl_pass(omhd_handle *h) {
   const omhd_config &c = h->cfg;
   k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
@@ -831,7 +896,7 @@ static int cfl_pass(omhd_handle *h) {
   int err[4];
   double vm;
   if (int rc = read_red(h, vm, err)) return rc;
-  if (err[0]) return state_error(err);
+  if (err[0]) return read_error(h, err);
   h->vmax = vm;
   return OMHD_OK;
 }
@@ -896,6 +961,7 @@ static Sweep base_sweep(omhd_handle *h, int pass) {
   p.acc = h->acc;
   p.vmax_bits = h->red;
   p.err = (int *)(h->red + 1);
+  p.err_next = (int *)(h->red + 3);
   return p;
 }
 
@@ -966,10 +1032,18 @@ extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_t
     // wavefront that overwrites it, just before it does)
     if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, dt, ch)) return rc;
     if (h->timing) OMHD_HIP(hipEventRecord(h->ev[2], h->stream));
-    int err[4];
+    int err[4], next[2];
     double vm;
-    if (int rc = read_red(h, vm, err)) return rc;
-    if (err[0]) return state_error(err);
+    if (int rc = read_red(h, vm, err, next)) return rc;
+    if (err[0]) return read_error(h, err);
+    if (next[0] || next[1]) {  // the step completed; the next u2v or set_dt stops on U^{n+1}
+      if (t) *t += dt;
+      if (dt_trace) dt_trace[n] = dt;
+      if (ch_trace) ch_trace[n] = ch;
+      if (done) *done = n + 1;
+      const int e[4] = {next[0] ? next[0] : next[1], 0, 0, 0};
+      return next[0] ? positivity_error(h, h->U, e) : state_error(e);
+    }
     h->vmax = vm;
     if (h->timing) {  // read_red synchronised the stream: both events are complete
       float a = 0.f, b = 0.f;
@@ -998,10 +1072,14 @@ extern "C" int omhd_step_fixed(omhd_handle *h, int32_t nsteps, const double *dt,
     if (int rc = stage_launch(h, 1, h->U, h->U, h->U1, dt[n], ch[n])) return rc;
     if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, dt[n], ch[n])) return rc;
   }
-  int err[4];
+  int err[4], next[2];
   double vm;
-  if (int rc = read_red(h, vm, err)) return rc;
-  if (err[0]) return state_error(err);
+  if (int rc = read_red(h, vm, err, next)) return rc;
+  if (err[0]) return read_error(h, err);
+  if (next[0] || next[1]) {
+    const int e[4] = {next[0] ? next[0] : next[1], 0, 0, 0};
+    return next[0] ? positivity_error(h, h->U, e) : state_error(e);
+  }
   if (int rc = cfl_pass(h)) return rc;  // the speed of the final state, for a later omhd_step
   return ok();
 }
@@ -1196,7 +1274,7 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
   if (e == hipSuccess) e = hipMalloc((void **)&b->arena, total * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&b->mem_d, n * sizeof(Sweep));
   if (e == hipSuccess) e = hipMalloc((void **)&b->vmax_d, n * sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMalloc((void **)&b->err_d, 4 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->err_d, 8 * sizeof(int));
   if (e == hipSuccess) e = hipMalloc((void **)&b->par_d, n * sizeof(DevPar));
   if (e == hipSuccess) e = hipMalloc((void **)&b->dx_d, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&b->cfl_d, n * sizeof(double));
@@ -1214,6 +1292,7 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
     p.U1 = p.U + (size_t)p.plane * NV;
     p.vmax_bits = b->vmax_d + i;
     p.err = b->err_d;
+    p.err_next = b->err_d + 4;
     p.dpar = (const double *)(b->par_d + i);
     dx[i] = cfgs[i].dx;
     cfl[i] = cfgs[i].cfl;
@@ -1222,7 +1301,7 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
   OMHD_HIP(hipMemcpy(b->dx_d, dx.data(), n * sizeof(double), hipMemcpyHostToDevice));
   OMHD_HIP(hipMemcpy(b->cfl_d, cfl.data(), n * sizeof(double), hipMemcpyHostToDevice));
   OMHD_HIP(hipMemcpy(b->tt_d, tt.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  OMHD_HIP(hipMemset(b->err_d, 0, 4 * sizeof(int)));
+  OMHD_HIP(hipMemset(b->err_d, 0, 8 * sizeof(int)));
   OMHD_HIP(hipMemset(b->tsum_d, 0, n * sizeof(double)));
   if (int rc = batch_tiles(b)) return rc;
   int dev = 0, cus = 256;
@@ -1248,10 +1327,15 @@ extern "C" int omhd_batch_destroy(omhd_batch *b) {
 }
 
 static int batch_check_err(omhd_batch *b) {
-  int err[4];
+  int err[8];
   OMHD_HIP(hipMemcpyAsync(err, b->err_d, sizeof err, hipMemcpyDeviceToHost, b->stream));
   OMHD_HIP(hipStreamSynchronize(b->stream));
-  if (err[0]) return state_error(err);
+  if (err[0]) {
+    err[0] &= ~ERR_IN_USTAR;
+    return state_error(err);
+  }
+  if (err[4]) return set_error(err[4], err[4] == OMHD_ERR_PRESSURE ? "negative pressure" : "negative density");
+  if (err[5]) return set_error(err[5], "non-finite signal speed");
   return OMHD_OK;
 }
 
@@ -1266,7 +1350,7 @@ extern "C" int omhd_batch_upload(omhd_batch *b, int32_t i, const double *U) {
   OMHD_HIP(hipGetLastError());
   a.release(b->stream);
   // first CFL speed of this member (main.f90:40)
-  OMHD_HIP(hipMemsetAsync(b->err_d, 0, 4 * sizeof(int), b->stream));
+  OMHD_HIP(hipMemsetAsync(b->err_d, 0, 8 * sizeof(int), b->stream));
   OMHD_HIP(hipMemsetAsync(b->vmax_d + i, 0, sizeof(unsigned long long), b->stream));
   k_cfl<2><<<(unsigned)(((size_t)p.nx * p.ny + 255) / 256), 256, 0, b->stream>>>(p.U, p.plane, p.zstride, p.pitch,
                                                                                   p.nx, p.ny, 1, b->vmax_d + i,

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, Config, check, lib
+from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, Config, OMHDError, check, lib
 from .api import KH_WALL_SIGN, _ptr
 
 
@@ -105,7 +105,11 @@ class Solver:
         rc = lib.omhd_step(self.h, nsteps, C.byref(t), dts.ctypes.data_as(DP) if trace else None,
                            chs.ctypes.data_as(DP) if trace else None, C.byref(done))
         self.t = t.value
-        check(rc, f"omhd_step (after {done.value} steps)")
+        try:
+            check(rc, f"omhd_step (after {done.value} steps)")
+        except OMHDError as e:
+            e.steps_done = done.value  # steps completed before the stop (the reference's stop)
+            raise
         return (dts, chs) if trace else None
 
     def state_shape(self):

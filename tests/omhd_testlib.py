@@ -172,6 +172,7 @@ class Oracle:
         chs = np.zeros(nsteps)
         bad = C.c_long(-1)
         rc = self.L.or_run(C.byref(prob), ptr(U), nsteps, ptr(dts), ptr(chs), C.byref(bad))
+        self.last_bad = bad.value  # 0-based linear cell of a failure (the reference's minloc)
         return rc, dts, chs
 
 
