@@ -101,3 +101,18 @@ def test_other_flux_types_oracle(oracle, typ, lm, tt, bc):
     U = smooth_state(np.random.default_rng(20 + typ), 66, 26, 18)
     dts, dto, Ug, Uo = run_both(oracle, U, 3, 0.1, lm, tt, bc=bc, rows=7, flux=typ)
     assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
+
+
+def test_harris_512x256x256_two_steps_vs_oracle(oracle):
+    """The 3D resistive Harris sheet at the benchmark's own shape (512x256x256;
+    each state buffer beyond 2 GiB, so 64-bit offsets everywhere): two full
+    steps, dt sequence and state bitwise."""
+    from openmhd_amd import problems
+
+    dx = 0.125
+    U, origin = problems.harris_sheet_3d(512, 256, 256, dx)
+    print("harris 512x256x256: built; running the oracle and the GPU", flush=True)
+    dts, dto, Ug, Uo = run_both(oracle, U, 2, dx, 1, 0, bc=("periodic", "wall", "periodic"),
+                                resistivity=(1e-3, 1e-2, 2.0), origin=origin)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
