@@ -90,3 +90,23 @@ def test_device_dt_graph_replay():
         rel = np.max(np.abs(Ue - Us)) / np.max(np.abs(Us))
         assert rel < 1e-10, (i, rel)
     e.close()
+
+
+def test_benchmark_ensemble_members_equal_solo_runs():
+    """The 256-member KH ensemble of the benchmark (sides 64-1024), host dt,
+    autotuned launch: 3 steps, every member's dt sequence and state bitwise
+    equal to its solo run."""
+    from openmhd_amd import Ensemble
+
+    cfgs, states = members(n=256, seed=2026, smin=64, smax=1024)
+    print(f"ensemble of {len(cfgs)} members built", flush=True)
+    e = Ensemble(cfgs)
+    for i, U in enumerate(states):
+        e.upload(i, U)
+    e.autotune(2)
+    tr = e.step(3, trace=True)
+    for i, (c, U) in enumerate(zip(cfgs, states)):
+        dts, Us = solo(c, U, 3)
+        assert bits_equal(tr[:, i], dts), i
+        assert bits_equal(e.download(i), Us), i
+    e.close()
