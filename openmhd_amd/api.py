@@ -18,6 +18,16 @@ from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, check, lib
 KH_WALL_SIGN = (1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0)
 
 
+def _fortran_strides(shape, strides) -> bool:
+    """True if the element strides are those of a dense column-major array."""
+    expect = 1
+    for size, stride in zip(shape, strides):
+        if size != 1 and stride != expect:
+            return False
+        expect *= size
+    return True
+
+
 def _ptr(a, n_expected: int | None = None):
     """Device or host pointer of a float64 array laid out in Fortran order."""
     if a is None:
@@ -27,8 +37,8 @@ def _ptr(a, n_expected: int | None = None):
 
         if a.dtype != torch.float64:
             raise TypeError("float64 tensor required")
-        if not (a.is_contiguous() or a.t().is_contiguous() or a.dim() > 2):
-            raise ValueError("contiguous storage required")
+        if not _fortran_strides(a.shape, a.stride()):
+            raise ValueError("Fortran-ordered (column-major) contiguous tensor required")
         if n_expected is not None and a.numel() < n_expected:
             raise ValueError(f"tensor has {a.numel()} elements, expected {n_expected}")
         return C.cast(C.c_void_p(a.data_ptr()), DP)

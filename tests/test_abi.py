@@ -44,3 +44,20 @@ def test_errors_without_gpu_are_reported_not_crashing():
     with pytest.raises(OMHDError) as e:
         api.limiter(np.zeros((4, 4), order="F"), np.zeros((4, 4), order="F"), np.zeros((4, 4), order="F"), 4, 4, 1, 9)
     assert e.value.code == 5  # OMHD_ERR_TYPE: the reference's 'unknown limiter' stop
+
+
+def test_row_major_tensor_is_rejected():
+    """A tensor that is not laid out column-major is refused before any call
+    into the library (the arrays are the reference's Fortran arrays)."""
+    import numpy as np
+    import pytest
+    import torch
+
+    from openmhd_amd import api
+
+    U = torch.zeros((6, 5, 9), dtype=torch.float64)  # C order: strides (45, 9, 1)
+    V = torch.from_numpy(np.zeros((6, 5, 4), order="F"))
+    with pytest.raises(ValueError):
+        api.u2v(U, V, 6, 5)
+    with pytest.raises(ValueError):
+        api.u2v(torch.from_numpy(np.zeros((6, 5, 9), order="F"))[:, ::2], V, 6, 5)  # a strided view
