@@ -88,7 +88,7 @@ def _function():
             s.set_state(W0.detach().contiguous())
             adj = Adjoint(s, roll.nsteps, roll.every)
             roll.dt, roll.ch = adj.forward(dt, ch)
-            WT = torch.empty_like(W0)
+            WT = torch.empty(W0.shape, dtype=W0.dtype, device=W0.device)  # dense, whatever W0's strides
             s.get_state(WT)  # synchronises the library's stream
             ctx.adj = adj
             return WT
