@@ -194,3 +194,15 @@ def test_other_flux_types_resident(typ, bc):
     s.close()
     assert bits_equal(dts, d[f"run_{typ}_{bc}_dt"]) and bits_equal(chs, d[f"run_{typ}_{bc}_ch"])
     assert bits_equal(Ug, d[f"run_{typ}_{bc}_U"])
+
+
+# tiny and ragged grids: sides of 2 (less than the two-cell halo), one past a
+# 60-column strip, chunks of one row, a single strip of 3 columns
+@pytest.mark.parametrize("nx,ny", ((2, 2), (3, 5), (61, 2), (2, 61), (119, 7), (60, 3)))
+@pytest.mark.parametrize("kh", (False, True))
+def test_tiny_and_ragged_grids(oracle, nx, ny, kh):
+    rng = np.random.default_rng(nx * 131 + ny)
+    U = smooth_state(rng, nx + 2, ny + 2)
+    dts, chs, Ug, dto, cho, Uo = run_both(oracle, U, 4, 0.05, 2, 0, kh=kh)
+    assert bits_equal(dts, dto) and bits_equal(chs, cho)
+    assert bits_equal(Ug, Uo)

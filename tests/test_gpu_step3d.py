@@ -116,3 +116,11 @@ def test_harris_512x256x256_two_steps_vs_oracle(oracle):
                                 resistivity=(1e-3, 1e-2, 2.0), origin=origin)
     assert bits_equal(dts, dto)
     assert bits_equal(Ug, Uo)
+
+
+@pytest.mark.parametrize("n", ((2, 2, 2), (3, 2, 5), (61, 3, 2), (2, 5, 61)))
+@pytest.mark.parametrize("walls", (("periodic",) * 3, ("wall", "wall", "wall")))
+def test_tiny_and_ragged_grids(oracle, n, walls):
+    U = smooth_state(np.random.default_rng(sum(n)), n[0] + 2, n[1] + 2, n[2] + 2)
+    dts, dto, Ug, Uo = run_both(oracle, U, 3, 0.1, 1, 0, bc=walls)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
