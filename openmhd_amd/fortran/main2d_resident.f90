@@ -48,6 +48,10 @@ program main2d_resident
      lm_type = 2
      if (kh) lm_type = 1
   endif
+  if (len_trim(fin) == 0) then
+     write(6,*) 'main2d_resident needs --input F (a snapshot in the format of 2D_basic/fileio.f90)'
+     stop 2
+  endif
   ix = n + 2; jx = n + 2
   allocate(U(ix,jx,9), V(ix,jx,4), x(ix), y(jx), dts(nsteps), chs(nsteps))
 
