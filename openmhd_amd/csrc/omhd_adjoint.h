@@ -561,71 +561,65 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
       const double at2 = Uh[mt2] / Uh[RO];
       double bat1 = 0, bat2 = 0;
       const bool useL = csign(0.5, aM) > 0;  // s1 = 0.5 + SIGN(0.5, aM) is 1 or 0
-      // side quantities of the side the flux takes
-      const double *S = useL ? L : R;
-      const double *US = useL ? UL : UR;
-      double *bUS = useL ? bUL : bUR;
-      double *bFS = useL ? bFL : bFR;
-      double *bS = useL ? bL : bR;
-      const double aS = useL ? aL : aR;
-      const double roS = useL ? roL : roR;
-      const double ptS = useL ? ptL : ptR;
-      const double vBS = useL ? vBL : vBR;
-      double &baS = useL ? baL : baR;
-      double &broS = useL ? broL : broR;
-      double &bptS = useL ? bptL : bptR;
-      double &bvBS = useL ? bvBL : bvBR;
-      const double P = vBS - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2];
-      const double Q = (aS - S[vn]) * US[EN] - ptS * S[vn] + pt * aM + Uh[bn] * P;
-      const double idS = 1.0 / (aS - aM);
-      const double enS = Q * idS;
-      // F[mn] = aS (roS aM - US[mn]) + FS[mn]
-      baS += (roS * aM - US[mn]) * bF[mn];
-      broS += aS * aM * bF[mn];
-      baM += aS * roS * bF[mn];
-      bUS[mn] -= aS * bF[mn];
-      bFS[mn] += bF[mn];
-      // F[mt1], F[mt2]
-      baS += (roS * at1 - US[mt1]) * bF[mt1] + (roS * at2 - US[mt2]) * bF[mt2];
-      broS += aS * at1 * bF[mt1] + aS * at2 * bF[mt2];
-      bat1 += aS * roS * bF[mt1];
-      bat2 += aS * roS * bF[mt2];
-      bUS[mt1] -= aS * bF[mt1];
-      bUS[mt2] -= aS * bF[mt2];
-      bFS[mt1] += bF[mt1];
-      bFS[mt2] += bF[mt2];
-      // F[EN] = aS (enS - US[EN]) + FS[EN]
-      baS += (enS - US[EN]) * bF[EN];
-      const double benS = aS * bF[EN];
-      bUS[EN] -= aS * bF[EN];
-      bFS[EN] += bF[EN];
-      // F[RO] = aS (roS - S[RO]) + FS[RO]
-      baS += (roS - S[RO]) * bF[RO];
-      broS += aS * bF[RO];
-      bS[RO] -= aS * bF[RO];
-      bFS[RO] += bF[RO];
-      // enS = Q / (aS - aM)
-      {
-        const double bQ = benS * idS;
-        const double bd = -bQ * enS;
-        baS += bd;
-        baM -= bd;
-        baS += US[EN] * bQ;
-        bS[vn] -= US[EN] * bQ + ptS * bQ;
-        bUS[EN] += (aS - S[vn]) * bQ;
-        bptS -= S[vn] * bQ;
-        bpt += aM * bQ;
-        baM += pt * bQ;
-        bUh[bn] += P * bQ;
-        const double bP = Uh[bn] * bQ;
-        bvBS += bP;
-        baM -= Uh[bn] * bP;
-        bUh[bn] -= aM * bP;
-        bat1 -= Uh[bt1] * bP;
-        bUh[bt1] -= at1 * bP;
-        bat2 -= Uh[bt2] * bP;
-        bUh[bt2] -= at2 * bP;
-      }
+      // the reverse of the side the flux takes, written once and inlined once
+      // per side, so every array it touches stays an ordinary register array
+      auto side = [&](const double *S, const double *US, double *bUS, double *bFS, double *bS, const double aS,
+                      const double roS, const double ptS, const double vBS, double &baS, double &broS,
+                      double &bptS, double &bvBS) __attribute__((always_inline)) {
+        const double P = vBS - aM * Uh[bn] - at1 * Uh[bt1] - at2 * Uh[bt2];
+        const double Q = (aS - S[vn]) * US[EN] - ptS * S[vn] + pt * aM + Uh[bn] * P;
+        const double idS = 1.0 / (aS - aM);
+        const double enS = Q * idS;
+        // F[mn] = aS (roS aM - US[mn]) + FS[mn]
+        baS += (roS * aM - US[mn]) * bF[mn];
+        broS += aS * aM * bF[mn];
+        baM += aS * roS * bF[mn];
+        bUS[mn] -= aS * bF[mn];
+        bFS[mn] += bF[mn];
+        // F[mt1], F[mt2]
+        baS += (roS * at1 - US[mt1]) * bF[mt1] + (roS * at2 - US[mt2]) * bF[mt2];
+        broS += aS * at1 * bF[mt1] + aS * at2 * bF[mt2];
+        bat1 += aS * roS * bF[mt1];
+        bat2 += aS * roS * bF[mt2];
+        bUS[mt1] -= aS * bF[mt1];
+        bUS[mt2] -= aS * bF[mt2];
+        bFS[mt1] += bF[mt1];
+        bFS[mt2] += bF[mt2];
+        // F[EN] = aS (enS - US[EN]) + FS[EN]
+        baS += (enS - US[EN]) * bF[EN];
+        const double benS = aS * bF[EN];
+        bUS[EN] -= aS * bF[EN];
+        bFS[EN] += bF[EN];
+        // F[RO] = aS (roS - S[RO]) + FS[RO]
+        baS += (roS - S[RO]) * bF[RO];
+        broS += aS * bF[RO];
+        bS[RO] -= aS * bF[RO];
+        bFS[RO] += bF[RO];
+        // enS = Q / (aS - aM)
+        {
+          const double bQ = benS * idS;
+          const double bd = -bQ * enS;
+          baS += bd;
+          baM -= bd;
+          baS += US[EN] * bQ;
+          bS[vn] -= US[EN] * bQ + ptS * bQ;
+          bUS[EN] += (aS - S[vn]) * bQ;
+          bptS -= S[vn] * bQ;
+          bpt += aM * bQ;
+          baM += pt * bQ;
+          bUh[bn] += P * bQ;
+          const double bP = Uh[bn] * bQ;
+          bvBS += bP;
+          baM -= Uh[bn] * bP;
+          bUh[bn] -= aM * bP;
+          bat1 -= Uh[bt1] * bP;
+          bUh[bt1] -= at1 * bP;
+          bat2 -= Uh[bt2] * bP;
+          bUh[bt2] -= at2 * bP;
+        }
+      };
+      if (useL) side(L, UL, bUL, bFL, bL, aL, roL, ptL, vBL, baL, broL, bptL, bvBL);
+      else side(R, UR, bUR, bFR, bR, aR, roR, ptR, vBR, baR, broR, bptR, bvBR);
       // at = Uh[mt] / Uh[RO]
       const double iro = 1.0 / Uh[RO];
       bUh[mt1] += bat1 * iro;
@@ -713,69 +707,65 @@ OMHD_HD void hlld_vjp(const double L[NV], const double R[NV], double ch, const d
         U2[bt2] = fc * Nb2;
         double broLs = 0, broRs = 0, bat1 = 0, bat2 = 0;
         const bool left = aM >= 0;
-        const double *S = left ? L : R;
-        const double *US = left ? UL : UR;
-        const double *US1 = left ? UL1 : UR1;
-        double *bUS = left ? bUL : bUR;
-        double *bUS1 = left ? bUL1 : bUR1;
-        double *bFS = left ? bFL : bFR;
-        double *bS = left ? bL : bR;
-        const double aS = left ? aL : aR;
-        const double aS1 = left ? aL1 : aR1;
-        const double roS = left ? roL : roR;
-        const double roSs = left ? roLs : roRs;
-        const double vt1S = left ? vt1L : vt1R, vt2S = left ? vt2L : vt2R;
-        const double sgn = left ? -1.0 : 1.0;  // L**: UL1[EN] - roLs(...)sB, R**: UR1[EN] + roRs(...)sB
-        double &baS = left ? baL : baR;
-        double &baS1 = left ? baL1 : baR1;
-        double &broS = left ? broL : broR;
-        double &broSs = left ? broLs : broRs;
-        double &bvt1S = left ? bvt1L : bvt1R;
-        double &bvt2S = left ? bvt2L : bvt2R;
-        U2[RO] = roS;
-        const double K = vt1S * US1[bt1] + vt2S * US1[bt2] - at1 * U2[bt1] - at2 * U2[bt2];
-        U2[EN] = US1[EN] + sgn * roSs * K * sB;
-        U2[mn] = roS * aM;
-        U2[mt1] = roS * at1;
-        U2[mt2] = roS * at2;
-        // F[m] = aS1 U2[m] - (aS1 - aS) US1[m] - aS XS[m] + FS[m]
-        for (int m = 0; m < 4; m++) {
-          baS1 += (U2[m] - US1[m]) * bF[m];
-          bU2[m] += aS1 * bF[m];
-          bUS1[m] -= (aS1 - aS) * bF[m];
-          baS += (US1[m] - US[m]) * bF[m];
-          bUS[m] -= aS * bF[m];
-          bFS[m] += bF[m];
-        }
-        for (int q = 0; q < 3; q++) {
-          const int m = q == 0 ? RO : (q == 1 ? bt1 : bt2);
-          baS1 += (U2[m] - US1[m]) * bF[m];
-          bU2[m] += aS1 * bF[m];
-          bUS1[m] -= (aS1 - aS) * bF[m];
-          baS += (US1[m] - S[m]) * bF[m];
-          bS[m] -= aS * bF[m];
-          bFS[m] += bF[m];
-        }
-        // U2[mt] = roS at, U2[mn] = roS aM, U2[RO] = roS
-        broS += at2 * bU2[mt2] + at1 * bU2[mt1] + aM * bU2[mn] + bU2[RO];
-        bat2 += roS * bU2[mt2];
-        bat1 += roS * bU2[mt1];
-        baM += roS * bU2[mn];
-        // U2[EN] = US1[EN] + sgn roSs K sB
-        {
-          const double b = bU2[EN];
-          bUS1[EN] += b;
-          broSs += sgn * K * sB * b;
-          const double bK = sgn * roSs * sB * b;
-          bvt1S += US1[bt1] * bK;
-          bUS1[bt1] += vt1S * bK;
-          bvt2S += US1[bt2] * bK;
-          bUS1[bt2] += vt2S * bK;
-          bat1 -= U2[bt1] * bK;
-          bU2[bt1] -= at1 * bK;
-          bat2 -= U2[bt2] * bK;
-          bU2[bt2] -= at2 * bK;
-        }
+        // the side whose double-star state the flux takes (L** or R**), inlined
+        // once per side so every array it touches stays a register array
+        auto central_side = [&](const double *S, const double *US, const double *US1, double *bUS, double *bUS1,
+                                double *bFS, double *bS, const double aS, const double aS1, const double roS,
+                                const double roSs, const double vt1S, const double vt2S, const double sgn,
+                                double &baS, double &baS1, double &broS, double &broSs, double &bvt1S,
+                                double &bvt2S) __attribute__((always_inline)) {
+          U2[RO] = roS;
+          const double K = vt1S * US1[bt1] + vt2S * US1[bt2] - at1 * U2[bt1] - at2 * U2[bt2];
+          U2[EN] = US1[EN] + sgn * roSs * K * sB;
+          U2[mn] = roS * aM;
+          U2[mt1] = roS * at1;
+          U2[mt2] = roS * at2;
+          // F[m] = aS1 U2[m] - (aS1 - aS) US1[m] - aS XS[m] + FS[m]
+          for (int m = 0; m < 4; m++) {
+            baS1 += (U2[m] - US1[m]) * bF[m];
+            bU2[m] += aS1 * bF[m];
+            bUS1[m] -= (aS1 - aS) * bF[m];
+            baS += (US1[m] - US[m]) * bF[m];
+            bUS[m] -= aS * bF[m];
+            bFS[m] += bF[m];
+          }
+          for (int q = 0; q < 3; q++) {
+            const int m = q == 0 ? RO : (q == 1 ? bt1 : bt2);
+            baS1 += (U2[m] - US1[m]) * bF[m];
+            bU2[m] += aS1 * bF[m];
+            bUS1[m] -= (aS1 - aS) * bF[m];
+            baS += (US1[m] - S[m]) * bF[m];
+            bS[m] -= aS * bF[m];
+            bFS[m] += bF[m];
+          }
+          // U2[mt] = roS at, U2[mn] = roS aM, U2[RO] = roS
+          broS += at2 * bU2[mt2] + at1 * bU2[mt1] + aM * bU2[mn] + bU2[RO];
+          bat2 += roS * bU2[mt2];
+          bat1 += roS * bU2[mt1];
+          baM += roS * bU2[mn];
+          // U2[EN] = US1[EN] + sgn roSs K sB
+          {
+            const double b = bU2[EN];
+            bUS1[EN] += b;
+            broSs += sgn * K * sB * b;
+            const double bK = sgn * roSs * sB * b;
+            bvt1S += US1[bt1] * bK;
+            bUS1[bt1] += vt1S * bK;
+            bvt2S += US1[bt2] * bK;
+            bUS1[bt2] += vt2S * bK;
+            bat1 -= U2[bt1] * bK;
+            bU2[bt1] -= at1 * bK;
+            bat2 -= U2[bt2] * bK;
+            bU2[bt2] -= at2 * bK;
+          }
+        };
+        // sgn: L** is UL1[EN] - roLs(...)sB, R** is UR1[EN] + roRs(...)sB
+        if (left)
+          central_side(L, UL, UL1, bUL, bUL1, bFL, bL, aL, aL1, roL, roLs, vt1L, vt2L, -1.0, baL, baL1, broL, broLs,
+                       bvt1L, bvt2L);
+        else
+          central_side(R, UR, UR1, bUR, bUR1, bFR, bR, aR, aR1, roR, roRs, vt1R, vt2R, 1.0, baR, baR1, broR, broRs,
+                       bvt1R, bvt2R);
         double bfc = 0;
         // U2[bt] = fc * Nb
         {
