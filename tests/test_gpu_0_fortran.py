@@ -83,3 +83,13 @@ def test_driver_1d_shock_tube_matches_oracle(tmp_path, oracle, flux, lm, tt):
     Vo = np.zeros((Uo.shape[0], 1, 4), order="F")
     assert oracle.u2v(Uo, Vo)[0] == 0
     assert bits_equal(Vg, Vo)
+
+
+# the 3D driver (3D_basic/main.f90's loop) through the 3D shims
+@pytest.mark.parametrize("lm,tt,flux", [(2, 0, 3), (1, 1, 3), (2, 0, 1), (4, 1, 2)])
+def test_driver_3d_matches_oracle(tmp_path, oracle, lm, tt, flux):
+    from test_fortran3d import run_main3d
+
+    dts, dto, Ug, Uo = run_main3d(FDIR / "main3d_gpu", tmp_path, oracle, lm, tt, flux)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
