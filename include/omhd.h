@@ -101,6 +101,17 @@ int omhd_bc_u(double *U, int32_t ix, int32_t jx, int32_t kx, const int32_t *bc_t
 int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int32_t dir, int32_t bc_type,
                  const double *sign_lo, const double *sign_hi);
 
+/* flux_resistive(F,U,x,y,dx,ix,jx,dir) -- 2D_reconnection/flux_resistive.f90:27,
+ * the 2D CPU form.  The caller evaluates its resistivity profile: eta_dx is an
+ * (ix,jx) column-major array of eta at the face centre times 1/dx, as :74 and
+ * :97 form it (eta(x(i)+dx/2, y(j))/dx for dir 1, eta(x(i), y(j)+dx/2)/dx for
+ * dir 2).  Adds the resistive terms to F(en), F(by) (dir 1) or F(bx) (dir 2),
+ * and F(bz) on the faces i=1..ix-1, j=2..jx-1 (dir 1) or i=2..ix-1, j=1..jx-1
+ * (dir 2); dir 3 adds nothing, as in the reference.  F(ix,jx,9), U(ix,jx,9)
+ * and eta_dx in host or device memory.  (set_dt_resistive, :18, is one line
+ * of scalar arithmetic and stays in the caller's language.) */
+int omhd_flux_resistive(double *F, const double *U, const double *eta_dx, int32_t ix, int32_t jx, int32_t dir);
+
 /* ---- resident step ------------------------------------------------------ */
 
 typedef struct omhd_config {

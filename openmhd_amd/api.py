@@ -12,10 +12,31 @@ import ctypes as C
 
 import numpy as np
 
+import math
+
 from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, check, lib
 
 # KH wall sign table, 2D_KH/bc.f90:17-36 and :61-81: my/vy and By negated.
 KH_WALL_SIGN = (1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0)
+
+
+def _negated(*slots):
+    s = [1.0] * 9
+    for m in slots:
+        s[m] = -1.0
+    return tuple(s)
+
+
+# 2D_reconnection/bc.f90: slots negated in the mirror ghosts, per face
+# (x-, x+, y-, y+, z-, z+): conducting walls in x (mx/vx, mz/vz, By, psi),
+# a symmetry plane at the south face (my/vy, mz/vz, Bx, psi), a wall at the
+# north face (my/vy, By).
+RECON_SIGN = (_negated(0, 2, 6, 8), _negated(0, 2, 6, 8), _negated(1, 2, 5, 8), _negated(1, 6), _negated(),
+              _negated())
+
+# resistivity of 2D_reconnection (flux_resistive.f90:5-7): 1/RM1 at the
+# X-point, 1/RM0 in the background
+RECON_RM1, RECON_RM0 = 60.0, 1000.0
 
 
 def _fortran_strides(shape, strides) -> bool:
@@ -150,6 +171,76 @@ def bc_for_F(VL, VR, ix, jx):
 def bc_for_G(VL, VR, ix, jx):
     """bc_for_G(VL,VR,ix,jx) -- 2D_KH/bc.f90:55 (wall face patch in y)."""
     _face(VL, VR, ix, jx, 1, 2, OMHD_BC_MIRROR, KH_WALL_SIGN, KH_WALL_SIGN)
+
+
+def bc_for_U_recon(U, ix, jx):
+    """bc_for_U(U,ix,jx) -- 2D_reconnection/bc.f90:6: reflecting faces with
+    the per-face tables of RECON_SIGN."""
+    _bc_u(U, ix, jx, 1, [OMHD_BC_MIRROR] * 4 + [OMHD_BC_PERIODIC] * 2, RECON_SIGN)
+
+
+def bc_for_F_recon(VL, VR, ix, jx):
+    """bc_for_F(VL,VR,ix,jx) -- 2D_reconnection/bc.f90:54 (x walls)."""
+    _face(VL, VR, ix, jx, 1, 1, OMHD_BC_MIRROR, RECON_SIGN[0], RECON_SIGN[1])
+
+
+def bc_for_G_recon(VL, VR, ix, jx):
+    """bc_for_G(VL,VR,ix,jx) -- 2D_reconnection/bc.f90:86 (south symmetry
+    plane, north wall)."""
+    _face(VL, VR, ix, jx, 1, 2, OMHD_BC_MIRROR, RECON_SIGN[2], RECON_SIGN[3])
+
+
+def eta_recon(xx: float, yy: float) -> float:
+    """eta(xx,yy) -- 2D_reconnection/flux_resistive.f90:1: 1/RM0 + (RM0-RM1)/
+    (RM0 RM1) cosh(min(r, 25))^-2, evaluated with the host's libm.  The
+    power -2 is formed as 1/(c*c), the way the reference's compiled build
+    evaluates it, so the value is bitwise the reference's."""
+    c = math.cosh(min(math.sqrt(xx * xx + yy * yy), 25.0))
+    return 1.0 / RECON_RM0 + (RECON_RM0 - RECON_RM1) / (RECON_RM0 * RECON_RM1) * (1.0 / (c * c))
+
+
+_eta_cache: dict = {}
+
+
+def eta_face_table(x, y, dx, dir, eta=eta_recon):
+    """The (ix,jx) Fortran-ordered table eta/dx at the faces of direction dir,
+    formed as flux_resistive.f90:74 (dir 1: eta(x(i)+dx/2, y(j)) * (1/dx)) and
+    :97 (dir 2: eta(x(i), y(j)+dx/2) * (1/dx)).  The last table is cached."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    key = (x.tobytes(), y.tobytes(), float(dx), int(dir), eta)
+    hit = _eta_cache.get("last")
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    odx = 1.0 / dx
+    t = np.empty((len(x), len(y)), order="F")
+    for j, yj in enumerate(y.tolist()):
+        for i, xi in enumerate(x.tolist()):
+            t[i, j] = (eta(xi + 0.5 * dx, yj) if dir == 1 else eta(xi, yj + 0.5 * dx)) * odx
+    _eta_cache["last"] = (key, t)
+    return t
+
+
+def flux_resistive(F, U, x, y, dx, ix, jx, dir, eta=eta_recon):
+    """flux_resistive(F,U,x,y,dx,ix,jx,dir) -- 2D_reconnection/
+    flux_resistive.f90:27: adds the resistive terms to F(en), F(by)/F(bx) and
+    F(bz) on the faces of direction dir (dir 3: nothing).  eta(xx, yy) is the
+    resistivity profile, evaluated on the host (default: that file's)."""
+    if dir not in (1, 2):
+        return
+    n = ix * jx
+    if len(x) != ix or len(y) != jx:
+        raise ValueError(f"x, y have {len(x)}, {len(y)} points, expected {ix}, {jx}")
+    t = eta_face_table(x, y, dx, dir, eta)
+    check(lib.omhd_flux_resistive(_ptr(F, 9 * n), _ptr(U, 9 * n), t.ctypes.data_as(DP), ix, jx, dir),
+          "flux_resistive")
+
+
+def set_dt_resistive(dt, dx, cfl, rm1=RECON_RM1):
+    """set_dt_resistive(dt,dx,cfl) -- 2D_reconnection/flux_resistive.f90:18:
+    returns min(dt, 0.5 cfl Rm1 dx^2), the explicit diffusion limit at the
+    resistivity peak."""
+    return min(dt, 0.5 * cfl * rm1 * (dx * dx))
 
 
 def periodic_patch(VL, VR, ix, jx, dir, kx=1):

@@ -227,6 +227,38 @@ __global__ void k_face_patch(double *__restrict__ VL, double *__restrict__ VR, i
   }
 }
 
+// 2D_reconnection/flux_resistive.f90:66-111, the 2D CPU form of the resistive
+// flux: face currents from cell-centred B (a 2-point normal difference and a
+// 4-point transverse average, Toth et al. 2008), with eta/dx at the face
+// folded into them before they enter F.  One lane per face over the faces
+// (i0..i0+ni-1, j0..j0+nj-1), 0-based; Fe points at F's plane en, so bx, by
+// and bz are planes 2, 3 and 4 from there; B points at U's plane bx.
+template <int D>
+__global__ __launch_bounds__(256) void k_flux_resistive2d(double *__restrict__ Fe, const double *__restrict__ B,
+                                                          const double *__restrict__ eta_dx, int ix, int i0, int j0,
+                                                          int ni, int nj, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ni * nj) return;
+  const size_t c = (size_t)(i0 + (int)(t % ni)) + (size_t)ix * (size_t)(j0 + (int)(t / ni));
+  const double *Bx = B, *By = B + n, *Bz = B + 2 * n;
+  const double e = eta_dx[c];
+  const size_t cn = c + ix;  // (i, j+1)
+  if (D == 1) {              // x-face (i+1/2, j): flux_resistive.f90:74-84
+    const size_t cs = c - ix;
+    const double JyS = -(e * (Bz[c + 1] - Bz[c]));
+    const double JzS = e * ((By[c + 1] - By[c]) - 0.25 * (((Bx[cn] + Bx[cn + 1]) - Bx[cs]) - Bx[cs + 1]));
+    Fe[c] = Fe[c] + 0.5 * (JyS * (Bz[c] + Bz[c + 1]) - JzS * (By[c] + By[c + 1]));
+    Fe[3 * n + c] = Fe[3 * n + c] - JzS;
+    Fe[4 * n + c] = Fe[4 * n + c] + JyS;
+  } else {  // y-face (i, j+1/2): :97-107
+    const double JxS = e * (Bz[cn] - Bz[c]);
+    const double JzS = e * (0.25 * (((By[cn + 1] + By[c + 1]) - By[cn - 1]) - By[c - 1]) - (Bx[cn] - Bx[c]));
+    Fe[c] = Fe[c] + 0.5 * (JzS * (Bx[c] + Bx[cn]) - JxS * (Bz[c] + Bz[cn]));
+    Fe[2 * n + c] = Fe[2 * n + c] + JzS;
+    Fe[4 * n + c] = Fe[4 * n + c] - JxS;
+  }
+}
+
 int check_ext(int ix, int jx, int kx) {
   if (ix < 1 || jx < 1 || kx < 1) return set_error(OMHD_ERR_EXTENT, "non-positive extent", ix, jx, kx);
   return OMHD_OK;
@@ -544,6 +576,36 @@ extern "C" int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int3
   if (int rc = r.finish(s)) return rc;
   l.release(s);
   r.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_flux_resistive(double *F, const double *U, const double *eta_dx, int32_t ix, int32_t jx,
+                                   int32_t dir) {
+  if (!F || !U || !eta_dx) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, 1)) return rc;
+  if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
+  // face ranges of flux_resistive.f90:49-55 (1-based i=1..ix-1, j=2..jx-1 for
+  // dir 1; i=2..ix-1, j=1..jx-1 for dir 2); dir 3 has no 2D term (:56-57)
+  const int i0 = dir == 1 ? 0 : 1, j0 = dir == 1 ? 1 : 0;
+  const int ni = ix - 1 - i0, nj = jx - 1 - j0;
+  if (dir == 3 || ni <= 0 || nj <= 0) return ok();
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx;
+  Staged f, b, e;  // only F(en..bz) and U(bx..bz) cross the bus
+  if (int rc = f.init(F + 3 * n, 5 * n, true, true, s)) return rc;
+  if (int rc = b.init(U + 5 * n, 3 * n, true, false, s)) return rc;
+  if (int rc = e.init(eta_dx, n, true, false, s)) return rc;
+  const unsigned nb = nblocks((size_t)ni * nj);
+  if (dir == 1)
+    k_flux_resistive2d<1><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, ix, i0, j0, ni, nj, n);
+  else
+    k_flux_resistive2d<2><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, ix, i0, j0, ni, nj, n);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = f.finish(s)) return rc;
+  f.release(s);
+  b.release(s);
+  e.release(s);
   OMHD_HIP(hipStreamSynchronize(s));
   return ok();
 }

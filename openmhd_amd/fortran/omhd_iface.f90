@@ -90,6 +90,12 @@ module omhd_iface
        integer(c_int32_t), value :: ix, jx, kx, dir, bc_type
        real(c_double), intent(in) :: sign_lo(9), sign_hi(9)
      end function
+     integer(c_int) function omhd_flux_resistive(F, U, eta_dx, ix, jx, dir) bind(C, name='omhd_flux_resistive')
+       import :: c_int, c_int32_t, c_double
+       real(c_double), intent(inout) :: F(*)
+       real(c_double), intent(in) :: U(*), eta_dx(*)
+       integer(c_int32_t), value :: ix, jx, dir
+     end function
      integer(c_int) function omhd_create(cfg, h) bind(C, name='omhd_create')
        import :: c_int, c_ptr, omhd_config
        type(omhd_config), intent(in) :: cfg

@@ -1,14 +1,17 @@
 !-----------------------------------------------------------------------
 ! Drop-in shims: the reference's step subroutines, same names and argument
 ! lists (2D forms), implemented by libomhd on the GPU.  A driver written
-! against the reference (2D_basic/main.f90, 2D_KH/main.f90) links against
-! these instead of the reference objects and runs unchanged.
+! against the reference (2D_basic/main.f90, 2D_KH/main.f90,
+! 2D_reconnection/main.f90) links against these instead of the reference
+! objects and runs unchanged.  The problem-specific routines sit in their own
+! files, as in the reference: bc_for_U/F/G of 2D_KH in omhd_shims_kh.f90;
+! bc_for_U/F/G, flux_resistive and set_dt_resistive of 2D_reconnection in
+! omhd_shims_recon.f90.
 !
 !   u2v          2D_basic/u2v.f90:1          limiter      2D_basic/limiter.f90:1
 !   flux_solver  2D_basic/flux_solver.f90:1  glm_ss2      2D_basic/glm_ss2.f90:1
 !   rk_tvd21/22, rk_std21/22  2D_basic/rk.f90:9,42,73,106
 !   set_dt       2D_basic/set_dt.f90:1       bc           2D_basic/bc.f90:1
-!   bc_for_U/F/G 2D_KH/bc.f90:6,42,55 (periodic x, walls in y)
 !-----------------------------------------------------------------------
 subroutine u2v(U, V, ix, jx)
   use omhd_iface
@@ -124,52 +127,3 @@ subroutine bc(U, ix, jx)
   rc = omhd_bc_u(U, ix, jx, 1, t, s)
   if (rc /= OMHD_OK) call omhd_fail(rc, 'bc')
 end subroutine bc
-
-! KH walls: my/vy and By change sign across the y faces (2D_KH/bc.f90:17-36)
-subroutine kh_wall_sign(s9)
-  implicit none
-  real(8), intent(out) :: s9(9)
-  s9 = 1.d0
-  s9(2) = -1.d0
-  s9(7) = -1.d0
-end subroutine kh_wall_sign
-
-subroutine bc_for_U(U, ix, jx)
-  use omhd_iface
-  implicit none
-  integer, intent(in) :: ix, jx
-  real(8), intent(inout) :: U(ix, jx, 9)
-  integer(c_int32_t) :: t(6)
-  real(8) :: s(9, 6)
-  integer(c_int) :: rc
-  t = (/ OMHD_BC_PERIODIC, OMHD_BC_PERIODIC, OMHD_BC_MIRROR, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, OMHD_BC_PERIODIC /)
-  s = 1.d0
-  call kh_wall_sign(s(:, 3))
-  call kh_wall_sign(s(:, 4))
-  rc = omhd_bc_u(U, ix, jx, 1, t, s)
-  if (rc /= OMHD_OK) call omhd_fail(rc, 'bc_for_U')
-end subroutine bc_for_U
-
-subroutine bc_for_F(VL, VR, ix, jx)
-  use omhd_iface
-  implicit none
-  integer, intent(in) :: ix, jx
-  real(8), intent(inout) :: VL(ix, jx, 9), VR(ix, jx, 9)
-  real(8) :: one(9)
-  integer(c_int) :: rc
-  one = 1.d0
-  rc = omhd_bc_face(VL, VR, ix, jx, 1, 1, OMHD_BC_PERIODIC, one, one)
-  if (rc /= OMHD_OK) call omhd_fail(rc, 'bc_for_F')
-end subroutine bc_for_F
-
-subroutine bc_for_G(VL, VR, ix, jx)
-  use omhd_iface
-  implicit none
-  integer, intent(in) :: ix, jx
-  real(8), intent(inout) :: VL(ix, jx, 9), VR(ix, jx, 9)
-  real(8) :: s(9)
-  integer(c_int) :: rc
-  call kh_wall_sign(s)
-  rc = omhd_bc_face(VL, VR, ix, jx, 1, 2, OMHD_BC_MIRROR, s, s)
-  if (rc /= OMHD_OK) call omhd_fail(rc, 'bc_for_G')
-end subroutine bc_for_G

@@ -168,6 +168,37 @@ def harris_sheet_3d(nx: int, ny: int, nz: int, dx: float, seed: int = 5, beta_b:
     return U, (x0, y0)
 
 
+def harris_sheet_2d(nx: int, ny: int, lx: float = 200.0, beta: float = 0.2, b1: float = 0.03):
+    """The 2D reconnection problem of 2D_reconnection/model.f90: a Harris
+    sheet Bx = tanh(y), rho = 1 + cosh(y)^-2 / beta, p = beta rho / 2, v = 0,
+    on x in [0, lx] and y from 0 (the south symmetry plane through the sheet),
+    with the X-point perturbation Bx -= b1 y exp(-r^2/4), By += b1 x
+    exp(-r^2/4) at the origin.  The grid is built the way model.f90:22-31
+    builds it (dx = lx / nx, x(1) = -dx/2, then x(i) = x(i-1) + dx; y alike
+    from -dx/2).  Returns (U, dx, x, y) with U(nx+2, ny+2, 9) in Fortran
+    order, ghosts included (a caller fills them with the problem's
+    bc_for_U before the first step)."""
+    ix, jx = nx + 2, ny + 2
+    dx = lx / float(ix - 2)
+    x = np.empty(ix)
+    y = np.empty(jx)
+    x[0] = -dx / 2
+    for i in range(1, ix):
+        x[i] = x[i - 1] + dx
+    y[0] = -dx / 2
+    for j in range(1, jx):
+        y[j] = y[j - 1] + dx
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    ro = 1.0 + np.cosh(np.minimum(Y, 25.0)) ** -2 / beta
+    p = 0.5 * beta * ro
+    g = np.exp(-(X ** 2 + Y ** 2) / 4.0)
+    bx = np.tanh(Y) - b1 * Y * g
+    by = b1 * X * g
+    z = np.zeros((ix, jx))
+    U = prim_to_cons_fortran(ro, z, z, z, p, bx, by, z, z)
+    return U, dx, x, y
+
+
 def kh_ensemble(n: int, seed: int = 2026, smin: int = 64, smax: int = 1024):
     """Parameters of an ensemble of independent KH runs drawn from one master
     seed: grid sides nx, ny log-uniform in [smin, smax] (independently), shear

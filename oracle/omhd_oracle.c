@@ -42,6 +42,10 @@
  *                     by closed forms (uniform B -> no flux, linear B -> const J)
  *   or_bc_u/or_bc_face per-face sign tables: flang vectors of 2D_KH/bc.f90 and
  *                     2D_reconnection/bc.f90
+ *   or_flux_resistive2d  the flang build of 2D_reconnection/flux_resistive.f90
+ *   or_set_dt_resistive2d  (bitwise vectors, with that file's eta); whole runs
+ *   (res_form 1)      of 2D_reconnection's loop against the project's driver
+ *                     linked with flang against the reference's routines
  */
 #include <math.h>
 #include <stdint.h>
@@ -871,6 +875,53 @@ double or_set_dt_resistive(double dt, double dx, double cfl, double eta_max)
 }
 
 /* ------------------------------------------------------------------------
+ * The 2D CPU form, 2D_reconnection/flux_resistive.f90:46-111: eta/dx at the
+ * face comes from the caller (eta_dx, an (ix,jx) array: eta(x(i)+dx/2, y(j))/dx
+ * on x-faces, :74, and eta(x(i), y(j)+dx/2)/dx on y-faces, :97) and is folded
+ * into the currents before they enter the flux.  Face ranges :49-55; dir 3
+ * adds nothing (:56-57).
+ * ---------------------------------------------------------------------- */
+void or_flux_resistive2d(double *F, const double *U, const double *eta_dx, int ix, int jx, int dir)
+{
+  const int kx = 1, k = 1;
+  int is, js;
+  if (dir == 1) { is = 1; js = 2; }
+  else if (dir == 2) { is = 2; js = 1; }
+  else return;
+  for (int j = js; j <= jx - 1; j++)
+    for (int i = is; i <= ix - 1; i++) {
+      const double e = eta_dx[(size_t)(i - 1) + (size_t)ix * (size_t)(j - 1)];
+#define B(ii, jj, m) U[IX4(ii, jj, k, m)]
+      if (dir == 1) {
+        double JyS = -(e * (B(i + 1, j, bz) - B(i, j, bz)));
+        double JzS = e * ((B(i + 1, j, by) - B(i, j, by)) -
+                          0.25 * (B(i, j + 1, bx) + B(i + 1, j + 1, bx) - B(i, j - 1, bx) - B(i + 1, j - 1, bx)));
+        F[IX4(i, j, k, en)] = F[IX4(i, j, k, en)] +
+                              0.5 * (JyS * (B(i, j, bz) + B(i + 1, j, bz)) - JzS * (B(i, j, by) + B(i + 1, j, by)));
+        F[IX4(i, j, k, by)] = F[IX4(i, j, k, by)] - JzS;
+        F[IX4(i, j, k, bz)] = F[IX4(i, j, k, bz)] + JyS;
+      } else {
+        double JxS = e * (B(i, j + 1, bz) - B(i, j, bz));
+        double JzS = e * (0.25 * (B(i + 1, j + 1, by) + B(i + 1, j, by) - B(i - 1, j + 1, by) - B(i - 1, j, by)) -
+                          (B(i, j + 1, bx) - B(i, j, bx)));
+        F[IX4(i, j, k, en)] = F[IX4(i, j, k, en)] +
+                              0.5 * (JzS * (B(i, j, bx) + B(i, j + 1, bx)) - JxS * (B(i, j, bz) + B(i, j + 1, bz)));
+        F[IX4(i, j, k, bx)] = F[IX4(i, j, k, bx)] + JzS;
+        F[IX4(i, j, k, bz)] = F[IX4(i, j, k, bz)] - JxS;
+      }
+#undef B
+    }
+}
+
+/* set_dt_resistive, 2D_reconnection/flux_resistive.f90:18-25:
+ * dt = min(dt, 0.5*cfl*Rm1*dx^2), Rm1 the resistivity peak's inverse. */
+double or_set_dt_resistive2d(double dt, double dx, double cfl, double rm1)
+{
+  double dr = 0.5 * cfl * rm1 * (dx * dx);
+  return dt < dr ? dt : dr;
+}
+
+/* ------------------------------------------------------------------------
  * One full step in the order of 2D_basic/main.f90:61-187 (2D) and
  * 3D_basic/main.f90 (3D):  u2v -> set_dt -> glm_ss2 -> per dimension
  * (9 x limiter -> face patch -> flux_solver) -> rk stage 1 -> bc(U1) ->
@@ -889,7 +940,10 @@ static int sweep(const or_problem *p, const double *Ust, const double *V, double
   int d = dir - 1;
   or_bc_face(VL, VR, ix, jx, kx, dir, p->bctype[2 * d], p->face_sgn + (2 * d) * 9, p->face_sgn + (2 * d + 1) * 9);
   if ((rc = or_flux_solver(FX, VL, VR, ch, ix, jx, kx, dir, p->flux_type, NULL))) return rc;
-  if (p->eta) or_flux_resistive(FX, Ust, p->eta + (size_t)d * ix * jx, p->dx, ix, jx, kx, dir); /* main.cuf:177-285 */
+  if (p->eta && p->res_form == 1) /* 2D_reconnection/main.f90:102,120,150,168 */
+    or_flux_resistive2d(FX, Ust, p->eta + (size_t)d * ix * jx, ix, jx, dir);
+  else if (p->eta)
+    or_flux_resistive(FX, Ust, p->eta + (size_t)d * ix * jx, p->dx, ix, jx, kx, dir); /* main.cuf:177-285 */
   return 0;
 }
 
@@ -913,7 +967,10 @@ int or_step(const or_problem *p, double *U, double *dt_out, double *ch_out, long
     ch = p->ch_fixed;
   } else {
     if ((rc = or_set_dt(U, V, &ch, &dt, p->dx, p->cfl, ix, jx, kx, nd, bad))) goto out;
-    if (p->eta) dt = or_set_dt_resistive(dt, p->dx, p->cfl, p->eta_max); /* main.cuf:147 */
+    if (p->eta && p->res_form == 1)
+      dt = or_set_dt_resistive2d(dt, p->dx, p->cfl, p->rm1); /* 2D_reconnection/main.f90:81 */
+    else if (p->eta)
+      dt = or_set_dt_resistive(dt, p->dx, p->cfl, p->eta_max); /* main.cuf:147 */
   }
   or_glm_ss2(U, ch, dt, ix, jx, kx);
   for (int stage = 0; stage < 2; stage++) {

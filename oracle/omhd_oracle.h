@@ -43,6 +43,10 @@ typedef struct or_problem {
   double ch_fixed;
   const double *eta;   /* NULL, or 3 face arrays (ix*jx each) of the resistivity */
   double eta_max;
+  int res_form;        /* 0: the 3D form (eta per face, dt limit 0.5 cfl dx^2/eta_max);
+                          1: the 2D CPU form of 2D_reconnection: eta holds 2 face
+                          arrays of eta/dx, dt limit 0.5 cfl rm1 dx^2 */
+  double rm1;          /* res_form 1: the Rm1 of set_dt_resistive */
 } or_problem;
 
 int or_u2v(const double *U, double *V, int ix, int jx, int kx, long *bad);
@@ -61,6 +65,8 @@ void or_bc_u(double *U, int ix, int jx, int kx, const int *bctype, const double 
 void or_bc_face(double *VL, double *VR, int ix, int jx, int kx, int dir, int bctype, const double *sgn_lo,
                 const double *sgn_hi);
 void or_flux_resistive(double *F, const double *U, const double *eta_face, double dx, int ix, int jx, int kx, int dir);
+void or_flux_resistive2d(double *F, const double *U, const double *eta_dx, int ix, int jx, int dir);
+double or_set_dt_resistive2d(double dt, double dx, double cfl, double rm1);
 double or_eta_faces(const double *x, const double *y, double dx, int ix, int jx, double eta0, double eta1, double w,
                     double *eta_out);
 double or_set_dt_resistive(double dt, double dx, double cfl, double eta_max);

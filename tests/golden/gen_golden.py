@@ -6,7 +6,7 @@ subroutines on crafted inputs and stores inputs + outputs as .npz (plain
 arrays, no pickles).  Branch labels for the HLLD vectors come from the oracle
 and are used only to choose inputs that cover every branch.
 
-    python tests/golden/gen_golden.py [3d | recon | snapshots | solvers]   (needs oracle/_ref, built here)
+    python tests/golden/gen_golden.py [3d | recon | recon_run | snapshots | solvers]   (needs oracle/_ref, built here)
 """
 from __future__ import annotations
 
@@ -343,11 +343,58 @@ def snapshots():
         print("wrote", name, name.stat().st_size, "bytes")
 
 
+def recon_run():
+    """2D_reconnection end to end: eta(x, y) and set_dt_resistive of
+    flux_resistive.f90 at sample points, and two short runs of the project's
+    driver (openmhd_amd/fortran/main2d_recon.f90) linked by flang against the
+    reference's 2D_basic + 2D_reconnection routines (oracle/_ref/
+    main2d_recon_ref): one CFL-limited, one limited by the resistive step."""
+    import ctypes as C
+    import subprocess
+    import tempfile
+
+    from omhd_testlib import REF_DIR, RefRec, recon_problem
+
+    from openmhd_amd import problems, snapshot
+
+    rr = RefRec()
+    rr.L.eta_.restype = C.c_double
+    rng = np.random.default_rng(45)
+    pts = np.concatenate([rng.uniform(-30, 30, (300, 2)), rng.uniform(-1, 1, (200, 2)),
+                          np.array([[0.0, 0.0], [-0.0, 0.0], [25.0, 0.0], [0.0, -25.0], [17.7, 17.7], [1e-8, 0.0]])])
+    eta = np.array([rr.L.eta_(C.byref(C.c_double(a)), C.byref(C.c_double(b))) for a, b in pts])
+    dtc = np.array([[1.0, 0.1, 0.4], [1e-4, 0.1, 0.4], [0.3, 0.01, 0.35], [2e-3, 0.0125, 0.8]])
+    dto = []
+    for dt, dx, cfl in dtc:
+        v = C.c_double(dt)
+        rr.L.set_dt_resistive_(C.byref(v), C.byref(C.c_double(dx)), C.byref(C.c_double(cfl)))
+        dto.append(v.value)
+    out = dict(eta_pts=pts, eta=eta, sdr_in=dtc, sdr_out=np.array(dto))
+    exe = REF_DIR / "main2d_recon_ref"
+    for name, lx, lm, tt, flux in (("a", 6.4, 1, 0, 3), ("b", 0.32, 2, 1, 3)):
+        nx, ny, steps = 32, 12, 8
+        U, _, x, y = problems.harris_sheet_2d(nx, ny, lx=lx)
+        p, tab = recon_problem(x, y, lm_type=lm, flux_type=flux, time_type=tt)
+        Oracle().bc_u(U, p)
+        with tempfile.TemporaryDirectory() as d:
+            fin, fo, tr = f"{d}/in.dat", f"{d}/out.dat", f"{d}/tr.bin"
+            snapshot.write(fin, 0.0, x, y, U)
+            subprocess.run([str(exe), str(nx), str(ny), str(steps), "--input", fin, "--output", fo, "--trace", tr,
+                            "--lm", str(lm), "--tt", str(tt), "--flux", str(flux)], check=True)
+            _, _, _, Uf, _ = snapshot.read(fo)
+            t = np.fromfile(tr)
+        out.update({f"{name}_U0": U, f"{name}_x": x, f"{name}_y": y, f"{name}_dt": t[:steps], f"{name}_ch": t[steps:],
+                    f"{name}_U": Uf, f"{name}_case": np.array([lm, tt, flux, steps])})
+    save("recon_run", **out)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "3d":
         only_3d()
     elif len(sys.argv) > 1 and sys.argv[1] == "recon":
         recon()
+    elif len(sys.argv) > 1 and sys.argv[1] == "recon_run":
+        recon_run()
     elif len(sys.argv) > 1 and sys.argv[1] == "snapshots":
         snapshots()
     elif len(sys.argv) > 1 and sys.argv[1] == "solvers":

@@ -47,6 +47,7 @@ class OrProblem(C.Structure):
         ("u_sgn", C.c_double * 54), ("face_sgn", C.c_double * 54),
         ("dt_fixed", C.c_double), ("ch_fixed", C.c_double),
         ("eta", DP), ("eta_max", C.c_double),
+        ("res_form", C.c_int), ("rm1", C.c_double),
     ]
 
 
@@ -279,6 +280,25 @@ def _neg(*slots):
 
 
 RECON_SIGN = [_neg(0, 2, 6, 8), _neg(0, 2, 6, 8), _neg(1, 2, 5, 8), _neg(1, 6), np.ones(9), np.ones(9)]
+
+
+def recon_problem(x, y, lm_type=1, flux_type=3, time_type=0, cfl=0.35):
+    """An oracle problem in the form of 2D_reconnection/main.f90: the per-face
+    tables of 2D_reconnection/bc.f90, the 2D CPU form of the resistive flux
+    with that file's eta(x, y) on the faces, and set_dt_resistive's limit.
+    dx = x(2) - x(1), as the drivers take it.  Returns (problem, tables); the
+    caller keeps `tables` alive while the problem is in use."""
+    from openmhd_amd.api import RECON_RM1, eta_face_table
+
+    ix, jx = len(x), len(y)
+    dx = float(x[1] - x[0])
+    p = make_problem(ix, jx, dx=dx, cfl=cfl, lm_type=lm_type, flux_type=flux_type, time_type=time_type,
+                     bc=("wall", "wall", "periodic"), signs=RECON_SIGN)
+    tables = np.ascontiguousarray(np.stack([eta_face_table(x, y, dx, d).T for d in (1, 2)]))
+    p.eta = ptr(tables)
+    p.res_form = 1
+    p.rm1 = RECON_RM1
+    return p, tables
 
 
 def ref_available(dim=2) -> bool:

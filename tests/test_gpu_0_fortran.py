@@ -1,5 +1,6 @@
 """The Fortran side of the boundary: the project's reference-style drivers
-(openmhd_amd/fortran/main2d.f90, main1d.f90) linked against the libomhd shims,
+(openmhd_amd/fortran/main1d.f90, main2d.f90, main2d_recon.f90, main3d.f90)
+linked against the libomhd shims,
 and the resident-step driver, all against the oracle on the same initial
 state.  Bitwise."""
 import subprocess
@@ -93,3 +94,39 @@ def test_driver_3d_matches_oracle(tmp_path, oracle, lm, tt, flux):
     dts, dto, Ug, Uo = run_main3d(FDIR / "main3d_gpu", tmp_path, oracle, lm, tt, flux)
     assert bits_equal(dts, dto)
     assert bits_equal(Ug, Uo)
+
+
+# the 2D reconnection driver (2D_reconnection/main.f90's loop) through the 2D
+# shims and omhd_shims_recon.f90: per-face reflecting tables, eta(x, y) face
+# tables, the resistive flux after each flux_solver call, set_dt_resistive
+@pytest.mark.parametrize("nx,ny,lx,steps,lm,tt,flux", [(64, 24, 12.8, 12, 1, 0, 3), (48, 20, 0.48, 10, 2, 1, 3),
+                                                      (40, 16, 8.0, 8, 3, 0, 1), (40, 16, 8.0, 8, 4, 1, 2),
+                                                      (40, 16, 8.0, 8, 0, 0, 0), (200, 50, 40.0, 40, 1, 0, 3)])
+def test_driver_recon_matches_oracle(tmp_path, oracle, nx, ny, lx, steps, lm, tt, flux):
+    from test_recon2d import run_recon_driver
+
+    dts, dto, Ug, Uo = run_recon_driver(FDIR / "main2d_recon_gpu", tmp_path, oracle, nx, ny, lx, steps, lm, tt, flux)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
+
+
+@pytest.mark.parametrize("case", ("a", "b"))
+def test_driver_recon_matches_reference_vectors(tmp_path, case):
+    """main2d_recon linked against the shims against the same driver linked
+    by flang with the reference's routines (tests/golden/recon_run.npz)."""
+    from openmhd_amd import snapshot
+    from test_oracle_golden import load
+
+    d = load("recon_run")
+    lm, tt, flux, steps = (int(v) for v in d[f"{case}_case"])
+    U0 = d[f"{case}_U0"]
+    fin, fout, tr = tmp_path / "in.dat", tmp_path / "out.dat", tmp_path / "trace.bin"
+    snapshot.write(fin, 0.0, d[f"{case}_x"], d[f"{case}_y"], np.asfortranarray(U0))
+    cmd = [str(FDIR / "main2d_recon_gpu"), str(U0.shape[0] - 2), str(U0.shape[1] - 2), str(steps), "--input", str(fin),
+           "--output", str(fout), "--trace", str(tr), "--lm", str(lm), "--tt", str(tt), "--flux", str(flux)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, _, Ug, _ = snapshot.read(fout)
+    t = np.fromfile(tr)
+    assert bits_equal(t[:steps], d[f"{case}_dt"]) and bits_equal(t[steps:], d[f"{case}_ch"])
+    assert bits_equal(Ug, d[f"{case}_U"])

@@ -9,7 +9,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from omhd_testlib import AdjCheck, KH_SIGN, cptr, make_problem, smooth_state
+from omhd_testlib import DP, AdjCheck, KH_SIGN, cptr, make_problem, smooth_state
 
 pytestmark = pytest.mark.gpu
 D = C.c_double
@@ -43,7 +43,7 @@ def setup(oracle, nx, ny, nz, bc, lm, tt, res, seed, dx=0.1, signs=None):
         y = snapshot.grid(ny, dx, origin[1])
         eta, emax = oracle.eta_faces(x, y, dx, *resistivity)
         eta = np.ascontiguousarray(eta)
-        p.eta = eta.ctypes.data_as(p._fields_[-2][1])
+        p.eta = eta.ctypes.data_as(DP)
         p.eta_max = emax
     oracle.bc_u(U, p)
     s = Solver(nx, ny, dx, nz=nz, cfl=0.3, lm_type=lm, time_type=tt, bc=bc, resistivity=resistivity, origin=origin,

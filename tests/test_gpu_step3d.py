@@ -3,7 +3,7 @@ against the oracle: bitwise dt sequence and state."""
 import numpy as np
 import pytest
 
-from omhd_testlib import make_problem, smooth_state
+from omhd_testlib import DP, make_problem, smooth_state
 from test_oracle_golden import bits_equal, load
 
 pytestmark = pytest.mark.gpu
@@ -20,7 +20,7 @@ def run_both(oracle, U, nsteps, dx, lm=2, tt=0, bc=("periodic",) * 3, rows=0, re
         x = snapshot.grid(ix - 2, dx, origin[0])
         y = snapshot.grid(jx - 2, dx, origin[1])
         eta, emax = oracle.eta_faces(x, y, dx, *resistivity)
-        p.eta = eta.ctypes.data_as(p._fields_[-2][1])
+        p.eta = eta.ctypes.data_as(DP)
         p.eta_max = emax
     oracle.bc_u(U, p)
     s = Solver(ix - 2, jx - 2, dx, nz=kx - 2, cfl=0.4, lm_type=lm, time_type=tt, bc=bc, rows_per_wave=rows,
