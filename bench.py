@@ -41,6 +41,15 @@ def parse_traffic():
     return float(m.group(1)) if m else None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
+    try:
+        names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs on the host)" if names else "unknown"
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline(n: int, sample_steps: int):
     """The reference CPU build (oracle/_ref, the project's Fortran driver over the
     reference's own routines, OpenMP) timed on this host on a bounded sample of
@@ -54,17 +63,27 @@ def cpu_baseline(n: int, sample_steps: int):
     def big_stack():  # set_dt/glm_ss2 keep ix*jx automatic arrays on the stack
         resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
-    if drv.exists():
-        cmd = [str(drv), "--kh", str(n), str(sample_steps + 1), "--time"]
-        log("cpu baseline:", " ".join(cmd))
-        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900, preexec_fn=big_stack)
+    def ref_rate(nthreads, steps):
+        cmd = [str(drv), "--kh", str(n), str(steps + 1), "--time"]
+        log(f"cpu baseline ({nthreads} threads):", " ".join(cmd))
+        out = subprocess.run(cmd, env=dict(env, OMP_NUM_THREADS=str(nthreads)), capture_output=True, text=True,
+                             timeout=900, preexec_fn=big_stack)
         m = re.search(r"zone_updates_per_s\s+([0-9.eE+]+)", out.stdout)
         if out.returncode == 0 and m:
-            return {"value": float(m.group(1)), "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+            return float(m.group(1))
+        log("reference driver failed:", out.stdout[-500:], out.stderr[-500:])
+        return None
+
+    if drv.exists():
+        v = ref_rate(threads, sample_steps)
+        if v is not None:
+            v1 = ref_rate(1, 1)
+            return {"value": v, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+                    "single_thread": v1, "cpu": cpu_model(),
                     "sample": f"KH {n}x{n} minmod, steps 2..{sample_steps + 1} of a {sample_steps + 1}-step run "
                               f"(driver's own KH layer init), the reference's routines compiled unmodified with "
-                              f"flang -O2 -fopenmp, {threads} OpenMP threads"}
-        log("reference driver failed:", out.stdout[-500:], out.stderr[-500:])
+                              f"flang -O2 -fopenmp, {threads} OpenMP threads; single_thread: step 2 of a 2-step "
+                              f"run on 1 thread"}
     # C restatement (port)
     sys.path.insert(0, str(REPO / "tests"))
     from omhd_testlib import Oracle, make_problem
@@ -80,6 +99,7 @@ def cpu_baseline(n: int, sample_steps: int):
     orc.run(p, U, sample_steps)
     dt = time.perf_counter() - t0
     return {"value": n * n * sample_steps / dt, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"KH {n}x{n} minmod, {sample_steps} steps after one untimed step, C restatement "
                       f"(gcc -O2 -fopenmp, {threads} threads)"}
 

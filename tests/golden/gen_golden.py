@@ -6,7 +6,7 @@ subroutines on crafted inputs and stores inputs + outputs as .npz (plain
 arrays, no pickles).  Branch labels for the HLLD vectors come from the oracle
 and are used only to choose inputs that cover every branch.
 
-    python tests/golden/gen_golden.py [3d | recon | recon_run | snapshots | solvers]   (needs oracle/_ref, built here)
+    python tests/golden/gen_golden.py [3d | recon | recon_run | ot_divb | snapshots | solvers]   (needs oracle/_ref, built here)
 """
 from __future__ import annotations
 
@@ -194,10 +194,13 @@ def main():
     U0sha = hashlib.sha256(U.tobytes(order="F")).hexdigest()
     dts, chs = ref_run_2d(ref2, U, 1000, dx, 0.4, 2)
     interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
+    sys.path.insert(0, str(HERE.parents[1]))
+    from openmhd_amd.diagnostics import div_b_l2  # noqa: E402
+
     save("ot200_mc_1000", dt=dts, ch=chs, dx=np.array(dx), u0_sha256=np.array(U0sha),
          final_sha256=np.array(hashlib.sha256(interior.tobytes()).hexdigest()),
          field_sum=interior.sum(axis=(1, 2)), field_absmax=np.abs(interior).max(axis=(1, 2)),
-         sample=interior[:, ::7, ::7])
+         sample=interior[:, ::7, ::7], divb_l2=np.array(div_b_l2(U, dx)))
 
     sys.path.insert(0, str(HERE.parents[1]))
     from openmhd_amd.problems import kelvin_helmholtz  # noqa: E402
@@ -343,6 +346,24 @@ def snapshots():
         print("wrote", name, name.stat().st_size, "bytes")
 
 
+def ot_divb():
+    """Adds the L2 norm of div B of the reference's OT 200^2 MC state after
+    1000 steps (openmhd_amd.diagnostics.div_b_l2) to ot200_mc_1000.npz,
+    after checking that the rerun reproduces the stored final-state hash."""
+    sys.path.insert(0, str(HERE.parents[1]))
+    from openmhd_amd.diagnostics import div_b_l2
+
+    d = dict(np.load(OUT / "ot200_mc_1000.npz", allow_pickle=False))
+    ref2 = Ref(2)
+    U, dx = orszag_tang(200)
+    ref2.bc(U)
+    ref_run_2d(ref2, U, 1000, dx, 0.4, 2)
+    interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
+    assert hashlib.sha256(interior.tobytes()).hexdigest() == str(d["final_sha256"])
+    d["divb_l2"] = np.array(div_b_l2(U, dx))
+    save("ot200_mc_1000", **d)
+
+
 def recon_run():
     """2D_reconnection end to end: eta(x, y) and set_dt_resistive of
     flux_resistive.f90 at sample points, and two short runs of the project's
@@ -395,6 +416,8 @@ if __name__ == "__main__":
         recon()
     elif len(sys.argv) > 1 and sys.argv[1] == "recon_run":
         recon_run()
+    elif len(sys.argv) > 1 and sys.argv[1] == "ot_divb":
+        ot_divb()
     elif len(sys.argv) > 1 and sys.argv[1] == "snapshots":
         snapshots()
     elif len(sys.argv) > 1 and sys.argv[1] == "solvers":

@@ -130,6 +130,9 @@ def test_orszag_tang_1000_steps(oracle):
     assert bits_equal(dts, d["dt"]) and bits_equal(chs, d["ch"])
     interior = np.ascontiguousarray(U[1:-1, 1:-1, :].transpose(2, 1, 0))
     assert hashlib.sha256(interior.tobytes()).hexdigest() == str(d["final_sha256"])
+    from openmhd_amd.diagnostics import div_b_l2
+
+    assert bits_equal(div_b_l2(U, dx), d["divb_l2"])  # the reference's ||div B||_2, from its own final state
 
 
 def test_3d_periodic_run(oracle):
