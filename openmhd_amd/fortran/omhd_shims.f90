@@ -31,6 +31,7 @@ subroutine limiter(wk, wL, wR, ix, jx, dir, type)
   real(8), intent(in) :: wk(ix, jx)
   real(8), intent(inout) :: wL(ix, jx), wR(ix, jx)
   integer(c_int) :: rc
+  if (dir /= 1 .and. dir /= 2) return   ! the 2D limiter.f90 acts on dir 1 and 2 only (:30-46)
   rc = omhd_limiter(wk, wL, wR, ix, jx, 1, dir, type)
   if (rc /= OMHD_OK) call omhd_fail(rc, 'limiter')
 end subroutine limiter
@@ -43,6 +44,8 @@ subroutine flux_solver(F, VL, VR, ch, ix, jx, dir, type)
   real(8), intent(in) :: VL(ix, jx, 9), VR(ix, jx, 9)
   real(8), intent(in) :: ch
   integer(c_int) :: rc
+  if (type < 0 .or. type > 4) return   ! no default case in flux_solver.f90:84-791: F stays as it is
+  if (dir /= 1 .and. dir /= 2) return   ! nor for dir (:73-81)
   rc = omhd_flux_solver(F, VL, VR, ch, ix, jx, 1, dir, type)
   if (rc /= OMHD_OK) call omhd_fail(rc, 'flux_solver')
 end subroutine flux_solver

@@ -29,6 +29,7 @@ subroutine limiter(wk, wL, wR, ix, jx, kx, dir, type)
   real(8), intent(in) :: wk(ix, jx, kx)
   real(8), intent(inout) :: wL(ix, jx, kx), wR(ix, jx, kx)
   integer(c_int) :: rc
+  if (dir < 1 .or. dir > 3) return   ! limiter.f90 has no default case for dir
   rc = omhd_limiter(wk, wL, wR, ix, jx, kx, dir, type)
   if (rc /= OMHD_OK) call omhd_fail3(rc, 'limiter')
 end subroutine limiter
@@ -41,6 +42,8 @@ subroutine flux_solver(F, VL, VR, ch, ix, jx, kx, dir, type)
   real(8), intent(in) :: VL(ix, jx, kx, 9), VR(ix, jx, kx, 9)
   real(8), intent(in) :: ch
   integer(c_int) :: rc
+  if (type < 0 .or. type > 4) return   ! no default case in flux_solver.f90:84-791: F stays as it is
+  if (dir < 1 .or. dir > 3) return   ! nor for dir
   rc = omhd_flux_solver(F, VL, VR, ch, ix, jx, kx, dir, type)
   if (rc /= OMHD_OK) call omhd_fail3(rc, 'flux_solver')
 end subroutine flux_solver
