@@ -112,6 +112,18 @@ int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int
  * of scalar arithmetic and stays in the caller's language.) */
 int omhd_flux_resistive(double *F, const double *U, const double *eta_dx, int32_t ix, int32_t jx, int32_t dir);
 
+/* flux_resistive(F,U,x,y,z,dx,ix,jx,kx,dir) -- 3D_reconnection_gpu/
+ * flux_resistive.cuf:12, the 3D form (a CUDA Fortran kernel in the reference;
+ * here callable from the host).  eta holds the resistivity at the face centres
+ * of direction dir (x(i)+dx/2 on x-faces, y(j)+dx/2 on y-faces, z(k)+dx/2 on
+ * z-faces, :47-68), an (ix,jx) array when it does not vary in z (eta_nz = 1,
+ * as the reference's profile) or (ix,jx,kx) (eta_nz = kx); each face divides
+ * it by dx (:83).  Adds the resistive terms to F(en) and the two transverse B
+ * slots on the faces of :47-68 (normal index 1..n-1, transverse 2..n-1).
+ * F(ix,jx,kx,9), U(ix,jx,kx,9), eta: host or device memory. */
+int omhd_flux_resistive3d(double *F, const double *U, const double *eta, int32_t eta_nz, double dx, int32_t ix,
+                          int32_t jx, int32_t kx, int32_t dir);
+
 /* ---- resident step ------------------------------------------------------ */
 
 typedef struct omhd_config {

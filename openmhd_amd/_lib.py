@@ -81,6 +81,7 @@ def _load() -> C.CDLL:
         "omhd_bc_u": [DP, I32, I32, I32, C.POINTER(I32), DP],
         "omhd_bc_face": [DP, DP, I32, I32, I32, I32, I32, DP, DP],
         "omhd_flux_resistive": [DP, DP, DP, I32, I32, I32],
+        "omhd_flux_resistive3d": [DP, DP, DP, I32, C.c_double, I32, I32, I32, I32],
         "omhd_create": [C.POINTER(Config), C.POINTER(C.c_void_p)],
         "omhd_upload": [C.c_void_p, DP],
         "omhd_step": [C.c_void_p, I32, DP, DP, DP, C.POINTER(I32)],

@@ -173,21 +173,29 @@ def bc_for_G(VL, VR, ix, jx):
     _face(VL, VR, ix, jx, 1, 2, OMHD_BC_MIRROR, KH_WALL_SIGN, KH_WALL_SIGN)
 
 
-def bc_for_U_recon(U, ix, jx):
+def bc_for_U_recon(U, ix, jx, kx=1):
     """bc_for_U(U,ix,jx) -- 2D_reconnection/bc.f90:6: reflecting faces with
-    the per-face tables of RECON_SIGN."""
-    _bc_u(U, ix, jx, 1, [OMHD_BC_MIRROR] * 4 + [OMHD_BC_PERIODIC] * 2, RECON_SIGN)
+    the per-face tables of RECON_SIGN; with kx > 1 the 3D form of
+    3D_reconnection_gpu/bc.cuf:7, periodic in z."""
+    _bc_u(U, ix, jx, kx, [OMHD_BC_MIRROR] * 4 + [OMHD_BC_PERIODIC] * 2, RECON_SIGN)
 
 
-def bc_for_F_recon(VL, VR, ix, jx):
-    """bc_for_F(VL,VR,ix,jx) -- 2D_reconnection/bc.f90:54 (x walls)."""
-    _face(VL, VR, ix, jx, 1, 1, OMHD_BC_MIRROR, RECON_SIGN[0], RECON_SIGN[1])
+def bc_for_F_recon(VL, VR, ix, jx, kx=1):
+    """bc_for_F(VL,VR,ix,jx) -- 2D_reconnection/bc.f90:54 (x walls);
+    3D_reconnection_gpu/bc.cuf:103."""
+    _face(VL, VR, ix, jx, kx, 1, OMHD_BC_MIRROR, RECON_SIGN[0], RECON_SIGN[1])
 
 
-def bc_for_G_recon(VL, VR, ix, jx):
+def bc_for_G_recon(VL, VR, ix, jx, kx=1):
     """bc_for_G(VL,VR,ix,jx) -- 2D_reconnection/bc.f90:86 (south symmetry
-    plane, north wall)."""
-    _face(VL, VR, ix, jx, 1, 2, OMHD_BC_MIRROR, RECON_SIGN[2], RECON_SIGN[3])
+    plane, north wall); 3D_reconnection_gpu/bc.cuf:147."""
+    _face(VL, VR, ix, jx, kx, 2, OMHD_BC_MIRROR, RECON_SIGN[2], RECON_SIGN[3])
+
+
+def bc_for_H_recon(VL, VR, ix, jx, kx):
+    """bc_for_H(VL,VR,ix,jx,kx) -- 3D_reconnection_gpu/bc.cuf:191 (periodic
+    z patch)."""
+    _face(VL, VR, ix, jx, kx, 3, OMHD_BC_PERIODIC, np.ones(9), np.ones(9))
 
 
 def eta_recon(xx: float, yy: float) -> float:
@@ -202,13 +210,16 @@ def eta_recon(xx: float, yy: float) -> float:
 _eta_cache: dict = {}
 
 
-def eta_face_table(x, y, dx, dir, eta=eta_recon):
-    """The (ix,jx) Fortran-ordered table eta/dx at the faces of direction dir,
-    formed as flux_resistive.f90:74 (dir 1: eta(x(i)+dx/2, y(j)) * (1/dx)) and
-    :97 (dir 2: eta(x(i), y(j)+dx/2) * (1/dx)).  The last table is cached."""
+def eta_face_table(x, y, dx, dir, eta=eta_recon, over_dx=True):
+    """The (ix,jx) Fortran-ordered table of eta at the face centres of
+    direction dir: eta(x(i)+dx/2, y(j)) on x-faces, eta(x(i), y(j)+dx/2) on
+    y-faces, eta(x(i), y(j)) on z-faces.  over_dx: times 1/dx, as the 2D form
+    forms eta/dx (flux_resistive.f90:74, :97); without it the plain values
+    the 3D form divides by dx itself (flux_resistive.cuf:47-83).  The last
+    table is cached."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
-    key = (x.tobytes(), y.tobytes(), float(dx), int(dir), eta)
+    key = (x.tobytes(), y.tobytes(), float(dx), int(dir), eta, bool(over_dx))
     hit = _eta_cache.get("last")
     if hit is not None and hit[0] == key:
         return hit[1]
@@ -216,7 +227,13 @@ def eta_face_table(x, y, dx, dir, eta=eta_recon):
     t = np.empty((len(x), len(y)), order="F")
     for j, yj in enumerate(y.tolist()):
         for i, xi in enumerate(x.tolist()):
-            t[i, j] = (eta(xi + 0.5 * dx, yj) if dir == 1 else eta(xi, yj + 0.5 * dx)) * odx
+            if dir == 1:
+                v = eta(xi + 0.5 * dx, yj)
+            elif dir == 2:
+                v = eta(xi, yj + 0.5 * dx)
+            else:
+                v = eta(xi, yj)
+            t[i, j] = v * odx if over_dx else v
     _eta_cache["last"] = (key, t)
     return t
 
@@ -236,8 +253,25 @@ def flux_resistive(F, U, x, y, dx, ix, jx, dir, eta=eta_recon):
           "flux_resistive")
 
 
+def flux_resistive_3d(F, U, x, y, z, dx, ix, jx, kx, dir, eta=eta_recon):
+    """flux_resistive(F,U,x,y,z,dx,ix,jx,kx,dir) -- 3D_reconnection_gpu/
+    flux_resistive.cuf:12, as a host call: adds the resistive terms to F(en)
+    and the two transverse B slots on the faces of direction dir.  eta(xx, yy)
+    is the resistivity profile (default: that file's, which does not vary in
+    z), evaluated on the host."""
+    if dir not in (1, 2, 3):
+        return
+    n = ix * jx * kx
+    if len(x) != ix or len(y) != jx or len(z) != kx:
+        raise ValueError(f"x, y, z have {len(x)}, {len(y)}, {len(z)} points, expected {ix}, {jx}, {kx}")
+    t = eta_face_table(x, y, dx, dir, eta, over_dx=False)
+    check(lib.omhd_flux_resistive3d(_ptr(F, 9 * n), _ptr(U, 9 * n), t.ctypes.data_as(DP), 1, dx, ix, jx, kx, dir),
+          "flux_resistive")
+
+
 def set_dt_resistive(dt, dx, cfl, rm1=RECON_RM1):
-    """set_dt_resistive(dt,dx,cfl) -- 2D_reconnection/flux_resistive.f90:18:
+    """set_dt_resistive(dt,dx,cfl) -- 2D_reconnection/flux_resistive.f90:18
+    (3D_reconnection_gpu/flux_resistive.cuf:2 alike):
     returns min(dt, 0.5 cfl Rm1 dx^2), the explicit diffusion limit at the
     resistivity peak."""
     return min(dt, 0.5 * cfl * rm1 * (dx * dx))

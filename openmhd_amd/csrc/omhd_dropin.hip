@@ -259,6 +259,48 @@ __global__ __launch_bounds__(256) void k_flux_resistive2d(double *__restrict__ F
   }
 }
 
+// 3D_reconnection_gpu/flux_resistive.cuf:86-135, the 3D form: face currents
+// from cell-centred B (2-point normal difference, 4-point transverse
+// averages), eta/dx formed per face as eta / dx (:83), then added to F.  One
+// lane per face over (i0.., j0.., k0..) with extents (ni, nj, nk), 0-based;
+// Fe points at F's plane en (bx, by, bz are planes 2, 3, 4 from there), B at
+// U's plane bx; eta holds eta at the faces, (ix,jx) when it does not vary in
+// z (eta_nz == 1) or (ix,jx,kx).
+template <int D>
+__global__ __launch_bounds__(256) void k_flux_resistive3d(double *__restrict__ Fe, const double *__restrict__ B,
+                                                          const double *__restrict__ eta, int eta_nz, double dx,
+                                                          int ix, int jx, int i0, int j0, int k0, int ni, int nj,
+                                                          int nk, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ni * nj * nk) return;
+  const int i = i0 + (int)(t % ni);
+  const size_t r = t / ni;
+  const int j = j0 + (int)(r % nj), k = k0 + (int)(r / nj);
+  const size_t jy = (size_t)ix, kz = (size_t)ix * jx;
+  const size_t c = (size_t)i + jy * j + kz * k;
+  const double *Bx = B, *By = B + n, *Bz = B + 2 * n;
+  const double eta_dx = eta[(size_t)i + jy * j + (eta_nz > 1 ? kz * k : 0)] / dx;
+  if (D == 1) {  // x-face (i+1/2, j, k): :89-101
+    const double JyS = -(Bz[c + 1] - Bz[c]) + 0.25 * (((Bx[c + kz] + Bx[c + 1 + kz]) - Bx[c - kz]) - Bx[c + 1 - kz]);
+    const double JzS = (By[c + 1] - By[c]) - 0.25 * (((Bx[c + jy] + Bx[c + 1 + jy]) - Bx[c - jy]) - Bx[c + 1 - jy]);
+    Fe[c] = Fe[c] + 0.5 * eta_dx * (JyS * (Bz[c] + Bz[c + 1]) - JzS * (By[c] + By[c + 1]));
+    Fe[3 * n + c] = Fe[3 * n + c] - eta_dx * JzS;
+    Fe[4 * n + c] = Fe[4 * n + c] + eta_dx * JyS;
+  } else if (D == 2) {  // y-face (i, j+1/2, k): :106-118
+    const double JxS = (Bz[c + jy] - Bz[c]) - 0.25 * (((By[c + kz] + By[c + jy + kz]) - By[c - kz]) - By[c + jy - kz]);
+    const double JzS = -(Bx[c + jy] - Bx[c]) + 0.25 * (((By[c + 1] + By[c + 1 + jy]) - By[c - 1]) - By[c - 1 + jy]);
+    Fe[c] = Fe[c] + 0.5 * eta_dx * (JzS * (Bx[c] + Bx[c + jy]) - JxS * (Bz[c] + Bz[c + jy]));
+    Fe[2 * n + c] = Fe[2 * n + c] + eta_dx * JzS;
+    Fe[4 * n + c] = Fe[4 * n + c] - eta_dx * JxS;
+  } else {  // z-face (i, j, k+1/2): :123-135
+    const double JyS = (Bx[c + kz] - Bx[c]) - 0.25 * (((Bz[c + 1] + Bz[c + 1 + kz]) - Bz[c - 1]) - Bz[c - 1 + kz]);
+    const double JxS = -(By[c + kz] - By[c]) + 0.25 * (((Bz[c + jy] + Bz[c + jy + kz]) - Bz[c - jy]) - Bz[c - jy + kz]);
+    Fe[c] = Fe[c] + 0.5 * eta_dx * (JxS * (By[c] + By[c + kz]) - JyS * (Bx[c] + Bx[c + kz]));
+    Fe[2 * n + c] = Fe[2 * n + c] - eta_dx * JyS;
+    Fe[3 * n + c] = Fe[3 * n + c] + eta_dx * JxS;
+  }
+}
+
 int check_ext(int ix, int jx, int kx) {
   if (ix < 1 || jx < 1 || kx < 1) return set_error(OMHD_ERR_EXTENT, "non-positive extent", ix, jx, kx);
   return OMHD_OK;
@@ -601,6 +643,39 @@ extern "C" int omhd_flux_resistive(double *F, const double *U, const double *eta
     k_flux_resistive2d<1><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, ix, i0, j0, ni, nj, n);
   else
     k_flux_resistive2d<2><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, ix, i0, j0, ni, nj, n);
+  OMHD_HIP(hipGetLastError());
+  if (int rc = f.finish(s)) return rc;
+  f.release(s);
+  b.release(s);
+  e.release(s);
+  OMHD_HIP(hipStreamSynchronize(s));
+  return ok();
+}
+
+extern "C" int omhd_flux_resistive3d(double *F, const double *U, const double *eta, int32_t eta_nz, double dx,
+                                     int32_t ix, int32_t jx, int32_t kx, int32_t dir) {
+  if (!F || !U || !eta) return set_error(OMHD_ERR_ARG, "null array");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (dir < 1 || dir > 3) return set_error(OMHD_ERR_DIR, "bad dir", dir);
+  if (eta_nz != 1 && eta_nz != kx) return set_error(OMHD_ERR_ARG, "eta_nz must be 1 or kx", eta_nz);
+  // face ranges of flux_resistive.cuf:47-68: the normal index from 1, the
+  // transverse ones over the interior (1-based)
+  const int i0 = dir == 1 ? 0 : 1, j0 = dir == 2 ? 0 : 1, k0 = dir == 3 ? 0 : 1;
+  const int ni = ix - 1 - i0, nj = jx - 1 - j0, nk = kx - 1 - k0;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return ok();
+  hipStream_t s = dropin_stream();
+  const size_t n = (size_t)ix * jx * kx;
+  Staged f, b, e;  // only F(en..bz) and U(bx..bz) cross the bus
+  if (int rc = f.init(F + 3 * n, 5 * n, true, true, s)) return rc;
+  if (int rc = b.init(U + 5 * n, 3 * n, true, false, s)) return rc;
+  if (int rc = e.init(eta, (size_t)ix * jx * eta_nz, true, false, s)) return rc;
+  const unsigned nb = nblocks((size_t)ni * nj * nk);
+  if (dir == 1)
+    k_flux_resistive3d<1><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, eta_nz, dx, ix, jx, i0, j0, k0, ni, nj, nk, n);
+  else if (dir == 2)
+    k_flux_resistive3d<2><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, eta_nz, dx, ix, jx, i0, j0, k0, ni, nj, nk, n);
+  else
+    k_flux_resistive3d<3><<<nb, TPB, 0, s>>>(f.dev, b.dev, e.dev, eta_nz, dx, ix, jx, i0, j0, k0, ni, nj, nk, n);
   OMHD_HIP(hipGetLastError());
   if (int rc = f.finish(s)) return rc;
   f.release(s);

@@ -96,6 +96,14 @@ module omhd_iface
        real(c_double), intent(in) :: U(*), eta_dx(*)
        integer(c_int32_t), value :: ix, jx, dir
      end function
+     integer(c_int) function omhd_flux_resistive3d(F, U, eta, eta_nz, dx, ix, jx, kx, dir) &
+          bind(C, name='omhd_flux_resistive3d')
+       import :: c_int, c_int32_t, c_double
+       real(c_double), intent(inout) :: F(*)
+       real(c_double), intent(in) :: U(*), eta(*)
+       integer(c_int32_t), value :: eta_nz, ix, jx, kx, dir
+       real(c_double), value :: dx
+     end function
      integer(c_int) function omhd_create(cfg, h) bind(C, name='omhd_create')
        import :: c_int, c_ptr, omhd_config
        type(omhd_config), intent(in) :: cfg

@@ -46,6 +46,9 @@
  *   or_set_dt_resistive2d  (bitwise vectors, with that file's eta); whole runs
  *   (res_form 1)      of 2D_reconnection's loop against the project's driver
  *                     linked with flang against the reference's routines
+ *   res_form 2        the 3D form above with set_dt_resistive's Rm1 limit
+ *                     (3D_reconnection_gpu/flux_resistive.cuf:2-9, the same
+ *                     expression as the 2D file's, pinned there)
  */
 #include <math.h>
 #include <stdint.h>
@@ -967,8 +970,8 @@ int or_step(const or_problem *p, double *U, double *dt_out, double *ch_out, long
     ch = p->ch_fixed;
   } else {
     if ((rc = or_set_dt(U, V, &ch, &dt, p->dx, p->cfl, ix, jx, kx, nd, bad))) goto out;
-    if (p->eta && p->res_form == 1)
-      dt = or_set_dt_resistive2d(dt, p->dx, p->cfl, p->rm1); /* 2D_reconnection/main.f90:81 */
+    if (p->eta && (p->res_form == 1 || p->res_form == 2)) /* 2D_reconnection/main.f90:81, main.cuf:147 */
+      dt = or_set_dt_resistive2d(dt, p->dx, p->cfl, p->rm1);
     else if (p->eta)
       dt = or_set_dt_resistive(dt, p->dx, p->cfl, p->eta_max); /* main.cuf:147 */
   }

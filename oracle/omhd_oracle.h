@@ -45,8 +45,10 @@ typedef struct or_problem {
   double eta_max;
   int res_form;        /* 0: the 3D form (eta per face, dt limit 0.5 cfl dx^2/eta_max);
                           1: the 2D CPU form of 2D_reconnection: eta holds 2 face
-                          arrays of eta/dx, dt limit 0.5 cfl rm1 dx^2 */
-  double rm1;          /* res_form 1: the Rm1 of set_dt_resistive */
+                          arrays of eta/dx, dt limit 0.5 cfl rm1 dx^2;
+                          2: the 3D form with the dt limit 0.5 cfl rm1 dx^2
+                          (3D_reconnection_gpu/flux_resistive.cuf:2-9) */
+  double rm1;          /* res_form 1, 2: the Rm1 of set_dt_resistive */
 } or_problem;
 
 int or_u2v(const double *U, double *V, int ix, int jx, int kx, long *bad);

@@ -301,6 +301,29 @@ def recon_problem(x, y, lm_type=1, flux_type=3, time_type=0, cfl=0.35):
     return p, tables
 
 
+def recon3d_problem(x, y, nz, lm_type=1, flux_type=3, time_type=0, cfl=0.3):
+    """An oracle problem in the form of 3D_reconnection_gpu/main.cuf: the
+    reflecting tables of bc.cuf in x and y, periodic z, the 3D resistive flux
+    with flux_resistive.cuf's eta(x, y) at the face centres (:47-83) and its
+    set_dt_resistive (:2-9).  dx = x(2) - x(1).  Returns (problem, tables)."""
+    from openmhd_amd.api import RECON_RM1, eta_recon
+
+    ix, jx = len(x), len(y)
+    dx = float(x[1] - x[0])
+    p = make_problem(ix, jx, nz + 2, dx=dx, cfl=cfl, lm_type=lm_type, flux_type=flux_type, time_type=time_type,
+                     bc=("wall", "wall", "periodic"), signs=RECON_SIGN)
+    t = np.empty((3, jx, ix))
+    for j in range(jx):
+        for i in range(ix):
+            t[0, j, i] = eta_recon(x[i] + 0.5 * dx, y[j])
+            t[1, j, i] = eta_recon(x[i], y[j] + 0.5 * dx)
+            t[2, j, i] = eta_recon(x[i], y[j])
+    p.eta = ptr(t)
+    p.res_form = 2
+    p.rm1 = RECON_RM1
+    return p, t
+
+
 def ref_available(dim=2) -> bool:
     return (REF_DIR / ("libref2d.so" if dim == 2 else "libref3d.so")).exists()
 

@@ -130,3 +130,42 @@ def test_driver_recon_matches_reference_vectors(tmp_path, case):
     t = np.fromfile(tr)
     assert bits_equal(t[:steps], d[f"{case}_dt"]) and bits_equal(t[steps:], d[f"{case}_ch"])
     assert bits_equal(Ug, d[f"{case}_U"])
+
+
+# the 3D reconnection driver (3D_reconnection_gpu/main.cuf's loop as host
+# calls) through omhd_shims3d.f90 and omhd_shims3d_recon.f90, against the
+# oracle's 3D resistive step with the same tables, eta and dt limit
+def run_recon3d_driver(tmp_path, oracle, n, dx, steps, lm, tt, flux, seed=11):
+    from omhd_testlib import recon3d_problem
+
+    from openmhd_amd import problems, snapshot
+
+    nx, ny, nz = n
+    U, (x0, y0) = problems.harris_sheet_3d(nx, ny, nz, dx, seed=seed, noise=1e-2)
+    x = snapshot.grid(nx, dx, x0)
+    y = snapshot.grid(ny, dx, y0)
+    z = snapshot.grid(nz, dx, 0.0)
+    p, tab = recon3d_problem(x, y, nz, lm_type=lm, flux_type=flux, time_type=tt)
+    oracle.bc_u(U, p)
+    fin, fout, tr = tmp_path / "in.dat", tmp_path / "out.dat", tmp_path / "trace.bin"
+    snapshot.write(fin, 0.0, x, y, U, z=z)
+    cmd = [str(FDIR / "main3d_recon_gpu"), *map(str, n), str(steps), "--input", str(fin), "--output", str(fout),
+           "--trace", str(tr), "--lm", str(lm), "--tt", str(tt), "--flux", str(flux)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, _, _, Ug, _ = snapshot.read(fout, three_d=True)
+    dts = np.fromfile(tr)[:steps]
+    Uo = U.copy(order="F")
+    rc, dto, _ = oracle.run(p, Uo, steps)
+    assert rc == 0
+    return dts, dto, Ug, Uo, float(p.dx)
+
+
+@pytest.mark.parametrize("n,dx,steps,lm,tt,flux", [((20, 16, 8), 0.25, 5, 1, 0, 3), ((18, 14, 6), 0.01, 4, 2, 1, 3),
+                                                   ((16, 12, 6), 0.2, 4, 1, 0, 1)])
+def test_driver_recon3d_matches_oracle(tmp_path, oracle, n, dx, steps, lm, tt, flux):
+    dts, dto, Ug, Uo, dxg = run_recon3d_driver(tmp_path, oracle, n, dx, steps, lm, tt, flux)
+    assert bits_equal(dts, dto)
+    assert bits_equal(Ug, Uo)
+    if dx < 0.05:  # the resistive limit sets dt (flux_resistive.cuf:8)
+        assert (dts == 0.5 * 0.3 * 60.0 * (dxg * dxg)).all()

@@ -1,7 +1,9 @@
-"""The 2D reconnection drop-in (omhd_flux_resistive, the reconnection
-boundary tables, the Python mirror of 2D_reconnection's routines) on the GPU
-against the oracle and the flang vectors of the reference's own routines:
-bitwise.  The driver-level runs are in tests/test_gpu_0_fortran.py."""
+"""The reconnection drop-ins on the GPU: the 2D form (omhd_flux_resistive,
+2D_reconnection's boundary tables, the Python mirror of its routines) against
+the oracle and the flang vectors of the reference's own routines, and the 3D
+form of 3D_reconnection_gpu (omhd_flux_resistive3d, bc.cuf's tables) against
+the oracle's restatement: bitwise.  The driver-level runs are in
+tests/test_gpu_0_fortran.py."""
 import ctypes as C
 
 import numpy as np
@@ -161,3 +163,98 @@ def test_api_reconnection_step_sequence_matches_oracle(oracle):
         api.bc_for_U_recon(U, ix, jx)
         api.glm_ss2(U, ch, dt, ix, jx)
     assert bits_equal(U, Uo)
+
+
+# ---------------------------------------------------------------------------
+# the 3D form (3D_reconnection_gpu/flux_resistive.cuf) as a host-callable
+# drop-in, omhd_flux_resistive3d, against the oracle's restatement
+# ---------------------------------------------------------------------------
+SIZES3 = [(12, 10, 8), (40, 23, 17), (3, 3, 3), (2, 7, 6), (7, 2, 6), (7, 6, 2), (3, 9, 3), (1, 4, 4)]
+
+
+@pytest.fixture(scope="module")
+def fr3d(oracle):
+    f = oracle.L.or_flux_resistive
+    f.argtypes = [DP, DP, DP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]
+    f.restype = None
+    return f
+
+
+@pytest.mark.parametrize("dir", (1, 2, 3))
+@pytest.mark.parametrize("ix,jx,kx", SIZES3)
+def test_flux_resistive3d_matches_oracle(lib, fr3d, ix, jx, kx, dir):
+    """Host arrays, a z-invariant face table of eta: bitwise to the oracle's
+    3D form, untouched faces and slots included."""
+    print(f"flux_resistive3d host {ix}x{jx}x{kx} dir={dir}", flush=True)
+    rng = np.random.default_rng(7 * ix + 11 * jx + 13 * kx + dir)
+    U = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+    F = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+    e = np.asfortranarray(rng.uniform(1e-3, 2e-2, size=(ix, jx)))
+    Fo = F.copy(order="F")
+    dx = 0.037
+    assert lib.omhd_flux_resistive3d(ptr(F), ptr(U), ptr(e), 1, dx, ix, jx, kx, dir) == 0
+    fr3d(ptr(Fo), ptr(U), ptr(e), dx, ix, jx, kx, dir)
+    assert bits_equal(F, Fo)
+
+
+@pytest.mark.parametrize("dir", (1, 2, 3))
+def test_flux_resistive3d_full_eta_table(lib, dir):
+    """eta_nz = kx reads one table per z plane: with F = 0 and plane k's
+    table 2^(k mod 3) times a base table, every increment scales exactly."""
+    ix, jx, kx, dx = 14, 11, 9, 0.05
+    rng = np.random.default_rng(40 + dir)
+    U = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+    e = np.asfortranarray(rng.uniform(1e-3, 2e-2, size=(ix, jx)))
+    scale = 2.0 ** (np.arange(kx) % 3)
+    full = np.asfortranarray(e[:, :, None] * scale[None, None, :])
+    F1, Fk = farr(ix, jx, kx, 9), farr(ix, jx, kx, 9)
+    assert lib.omhd_flux_resistive3d(ptr(F1), ptr(U), ptr(e), 1, dx, ix, jx, kx, dir) == 0
+    assert lib.omhd_flux_resistive3d(ptr(Fk), ptr(U), ptr(full), kx, dx, ix, jx, kx, dir) == 0
+    assert np.abs(F1).max() > 0
+    assert bits_equal(Fk, F1 * scale[None, None, :, None])
+
+
+def test_flux_resistive3d_argument_errors(lib):
+    from openmhd_amd._lib import OMHD_ERR_ARG, OMHD_ERR_DIR
+
+    U, F = farr(6, 5, 4, 9), farr(6, 5, 4, 9)
+    e = farr(6, 5)
+    assert lib.omhd_flux_resistive3d(ptr(F), ptr(U), ptr(e), 1, 0.1, 6, 5, 4, 0) == OMHD_ERR_DIR
+    assert lib.omhd_flux_resistive3d(ptr(F), ptr(U), ptr(e), 2, 0.1, 6, 5, 4, 1) == OMHD_ERR_ARG
+    assert lib.omhd_flux_resistive3d(ptr(F), None, ptr(e), 1, 0.1, 6, 5, 4, 1) == OMHD_ERR_ARG
+
+
+def test_api_flux_resistive_3d_and_boundaries_match_oracle(oracle, fr3d):
+    """The Python mirror's 3D reconnection routines: flux_resistive_3d with
+    flux_resistive.cuf's eta(x, y) against the oracle's 3D form, and the
+    bc.cuf tables (bc_for_U/F/G/H) against the oracle's, bitwise."""
+    from omhd_testlib import RECON_SIGN, make_problem
+
+    from openmhd_amd import api
+
+    ix, jx, kx, dx = 16, 12, 7, 0.3
+    x = (np.arange(ix) - ix / 2 + 0.5) * dx
+    y = (np.arange(jx) - 0.5) * dx
+    z = np.arange(kx) * dx
+    rng = np.random.default_rng(5)
+    U = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+    for dir in (1, 2, 3):
+        F = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+        Fo = F.copy(order="F")
+        api.flux_resistive_3d(F, U, x, y, z, dx, ix, jx, kx, dir)
+        fr3d(ptr(Fo), ptr(U), ptr(api.eta_face_table(x, y, dx, dir, over_dx=False)), dx, ix, jx, kx, dir)
+        assert np.abs(F - Fo).max() == 0 and bits_equal(F, Fo)
+    p = make_problem(ix, jx, kx, bc=("wall", "wall", "periodic"), signs=RECON_SIGN)
+    Ug, Uo = U.copy(order="F"), U.copy(order="F")
+    api.bc_for_U_recon(Ug, ix, jx, kx)
+    oracle.bc_u(Uo, p)
+    assert bits_equal(Ug, Uo)
+    for d, fn in ((1, api.bc_for_F_recon), (2, api.bc_for_G_recon), (3, api.bc_for_H_recon)):
+        VL = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+        VR = np.asfortranarray(rng.normal(size=(ix, jx, kx, 9)))
+        VLo, VRo = VL.copy(order="F"), VR.copy(order="F")
+        fn(VL, VR, ix, jx, kx)
+        lo = np.ascontiguousarray(RECON_SIGN[2 * d - 2])
+        hi = np.ascontiguousarray(RECON_SIGN[2 * d - 1])
+        oracle.L.or_bc_face(ptr(VLo), ptr(VRo), ix, jx, kx, d, 0 if d == 3 else 1, ptr(lo), ptr(hi))
+        assert bits_equal(VL, VLo) and bits_equal(VR, VRo), d

@@ -4,7 +4,7 @@ of the Python mirror, and whole runs of 2D_reconnection/main.f90's loop,
 against vectors from the flang build of the reference's own routines
 (tests/golden/recon.npz, recon_run.npz) and, where that build is present
 here, against the reference driver directly.  CPU only; the GPU side is in
-tests/test_gpu_recon2d.py and tests/test_gpu_0_fortran.py."""
+tests/test_gpu_recon.py and tests/test_gpu_0_fortran.py."""
 import ctypes as C
 import subprocess
 
