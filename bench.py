@@ -41,6 +41,28 @@ def parse_traffic():
     return float(m.group(1)) if m else None
 
 
+def stream_copy_gbs(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """What HBM delivers to a plain device copy on this GPU: bytes read plus
+    bytes written per second, over `reps` copies of `nbytes` timed with HIP
+    events (after two untimed copies)."""
+    import torch
+
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def cpu_model() -> str:
     """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
     try:
@@ -434,6 +456,8 @@ def main():
     achieved = BYTES_STAGE[dom + 1] * cells / (st[dom] * 1e-3) / 1e9
     traffic = parse_traffic()
     log(f"{ms_step:.3f} ms/step, {zups / 1e9:.3f} G zone-updates/s; stage1 {st[0]:.3f} ms, stage2 {st[1]:.3f} ms")
+    copy_gbs = stream_copy_gbs()
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
     out = {
         "metric": "FP64 zone-updates/s & %HBM peak; batched and adjoint time vs plain GPU step",
         "value": zups,
@@ -453,7 +477,8 @@ def main():
         "hbm_frac_step": zups * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": f"k_march stage {dom + 1} (k_march<LIM,{dom + 1},TT,0,0>)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_ms": st[dom], "algorithmic_bytes_per_launch": BYTES_STAGE[dom + 1] * cells},
+                     "avg_launch_ms": st[dom], "algorithmic_bytes_per_launch": BYTES_STAGE[dom + 1] * cells,
+                     "stream_copy": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
     }
     s.close()
