@@ -139,3 +139,68 @@ def test_rollout_autograd_function():
     W2 = torch.tensor(W0, device="cuda", requires_grad=True)
     WT2 = roll(W2, dt=roll.dt, ch=roll.ch)
     assert torch.equal(WT2.detach(), WT.detach())
+
+
+def oracle_L(oracle, W, dts, chs, lamT, dx, lm):
+    """<lamT, W_T> after the steps of the oracle (the CPU build's arithmetic,
+    bitwise the reference) from the interior state W, periodic, with the
+    given frozen (dt, ch) per step."""
+    import ctypes as C
+
+    from omhd_testlib import ptr
+
+    n = W.shape[1]
+    U = np.zeros((n + 2, n + 2, 9), order="F")
+    U[1:-1, 1:-1, :] = W.transpose(2, 1, 0)
+    p = make_problem(n + 2, n + 2, dx=dx, cfl=0.4, lm_type=lm)
+    oracle.bc_u(U, p)
+    dt, ch, bad = C.c_double(), C.c_double(), C.c_long(-1)
+    for a, b in zip(dts, chs):
+        p.dt_fixed, p.ch_fixed = float(a), float(b)
+        assert oracle.L.or_step(C.byref(p), ptr(U), C.byref(dt), C.byref(ch), C.byref(bad)) == 0
+    return float((lamT * soa(U)).sum())
+
+
+@pytest.mark.parametrize("lm", (1, 2))
+def test_adjoint_against_differences_of_the_cpu_build(oracle, lm):
+    """The reference gradient by finite differences of the CPU build: central
+    differences of the oracle's step (bitwise the reference's arithmetic) with
+    the adjoint run's frozen (dt, ch), on the case of the test above.  They
+    equal the differences of the GPU forward bit for bit (the two forward
+    maps are the same function of the state), and the GPU adjoint's
+    <dL/dW0, v> matches them.  How close differences can come is set by the
+    forward's rounding noise over 2 eps |<dL/dW0, v>|: with a smaller
+    directional derivative than here (other seeds, smaller grids) the best
+    eps leaves 1e-4..1e-3, while the exact tangent check above stays at
+    1e-12."""
+    from openmhd_amd import Adjoint, Solver
+
+    n, nsteps = 32, 10
+    _, W, dx = ot(n)
+    W0 = W + 1e-2 * smooth_field(np.random.default_rng(9), n)
+    s = Solver(n, n, dx, lm_type=lm)
+    s.set_state(W0)
+    adj = Adjoint(s, nsteps, 4)
+    dts, chs = adj.forward()
+    rng = np.random.default_rng(5)
+    lamT = smooth_field(rng, n)
+    grad = adj.backward(lamT)
+    best = []
+    for k in range(4):
+        v = smooth_field(rng, n)
+        an = float((grad * v).sum())
+        rel = []
+        for eps in (1e-7, 1e-8, 3e-9):
+            Lc = [oracle_L(oracle, W0 + sg * eps * v, dts, chs, lamT, dx, lm) for sg in (1, -1)]
+            Lg = []
+            for sg in (1, -1):
+                s.set_state(W0 + sg * eps * v)
+                s.step_fixed(dts, chs)
+                Lg.append(float((lamT * s.get_state()).sum()))
+            assert Lc == Lg, (k, eps)  # the CPU build's differences are the GPU forward's
+            rel.append(abs((Lc[0] - Lc[1]) / (2 * eps) - an) / abs(an))
+        best.append(min(rel))
+    print(f"lm {lm}: adjoint vs central differences of the CPU build, best eps per direction: {best}")
+    assert max(best) <= 1e-5 and float(np.median(best)) <= 1e-6, best
+    adj.close()
+    s.close()
