@@ -258,3 +258,38 @@ def test_api_flux_resistive_3d_and_boundaries_match_oracle(oracle, fr3d):
         hi = np.ascontiguousarray(RECON_SIGN[2 * d - 1])
         oracle.L.or_bc_face(ptr(VLo), ptr(VRo), ix, jx, kx, d, 0 if d == 3 else 1, ptr(lo), ptr(hi))
         assert bits_equal(VL, VLo) and bits_equal(VR, VRo), d
+
+
+def _dev_ptr(t):
+    return C.cast(C.c_void_p(t.data_ptr()), DP)
+
+
+@pytest.mark.parametrize("shape", [(4098, 4098, 1), (514, 258, 258)])
+def test_flux_resistive_full_size_device(lib, fr2d, fr3d, shape):
+    """At the benchmark grids (KH 4096^2 for the 2D form, the 3D resistive
+    Harris grid 512x256x256 for the 3D form, whose state planes exceed 2^31
+    bytes), on device tensors in place: bitwise to the oracle, every dir."""
+    import torch
+
+    ix, jx, kx = shape
+    rng = np.random.default_rng(ix + kx)
+    full = (ix, jx, 9) if kx == 1 else (ix, jx, kx, 9)
+    U = np.asfortranarray(rng.standard_normal(full))
+    e = np.asfortranarray(rng.uniform(1e-3, 2e-2, size=(ix, jx)))
+    dU, de = torch.from_numpy(U).to("cuda"), torch.from_numpy(e).to("cuda")
+    for dir in ((1, 2) if kx == 1 else (1, 2, 3)):
+        print(f"flux_resistive full size {shape} dir={dir}", flush=True)
+        F = np.asfortranarray(rng.standard_normal(full))
+        dF = torch.from_numpy(F).to("cuda")
+        if kx == 1:
+            rc = lib.omhd_flux_resistive(_dev_ptr(dF), _dev_ptr(dU), _dev_ptr(de), ix, jx, dir)
+            fr2d(ptr(F), ptr(U), ptr(e), ix, jx, dir)
+        else:
+            rc = lib.omhd_flux_resistive3d(_dev_ptr(dF), _dev_ptr(dU), _dev_ptr(de), 1, 0.0125, ix, jx, kx, dir)
+            fr3d(ptr(F), ptr(U), ptr(e), 0.0125, ix, jx, kx, dir)
+        torch.cuda.synchronize()
+        assert rc == 0
+        assert bits_equal(dF.cpu().numpy(), F)
+        del dF
+    del dU, de
+    torch.cuda.empty_cache()
