@@ -177,8 +177,10 @@ class Ensemble:
     launch per RK stage over a tile table of all members, each member with its
     own CFL dt.  configs: a list of omhd_config (make_config(...)); members share
     the limiter and time type.  device_dt=False: every member is bitwise equal
-    to its solo Solver run; device_dt=True (+ graph=True): dt computed on the
-    GPU, whole steps captured in a hipGraph."""
+    to its solo Solver run; device_dt=True (+ graph=True): dt and the GLM
+    factor computed on the GPU, whole steps captured in a hipGraph (the
+    device's exp can differ from the host libm's in the last bit, so members
+    then match their solo runs to about 1e-10 rather than bitwise)."""
 
     def __init__(self, configs):
         self.n = len(configs)
