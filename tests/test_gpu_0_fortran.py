@@ -169,3 +169,42 @@ def test_driver_recon3d_matches_oracle(tmp_path, oracle, n, dx, steps, lm, tt, f
     assert bits_equal(Ug, Uo)
     if dx < 0.05:  # the resistive limit sets dt (flux_resistive.cuf:8)
         assert (dts == 0.5 * 0.3 * 60.0 * (dxg * dxg)).all()
+
+
+# The reference's own programs, unmodified (1D_basic and 2D_basic: main.f90,
+# model.f90 and the output routine), built by oracle/Makefile with the
+# reference's routines (refprog_*_cpu) and with the libomhd shims
+# (refprog_*_gpu).  2D_basic is the Orszag-Tang run to t = 4 (about 840 steps,
+# 41 snapshot files in the fileio.f90 format).
+REFPROG = REPO / "oracle" / "_ref"
+
+
+def _big_stack():  # set_dt/glm_ss2 keep automatic arrays on the stack
+    import resource
+
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+@pytest.mark.parametrize("ex", ("1d_basic", "2d_basic"))
+def test_reference_program_unchanged_on_the_gpu(tmp_path, ex):
+    """The same program writes the same files, byte for byte, and the same
+    standard output, whether it is linked against the reference's routines
+    (on the CPU) or against the drop-in shims (on the GPU)."""
+    import filecmp
+
+    cpu, gpu = REFPROG / f"refprog_{ex}_cpu", REFPROG / f"refprog_{ex}_gpu"
+    if not (cpu.exists() and gpu.exists()):
+        pytest.skip("reference programs not built (make -C oracle ref)")
+    runs = {}
+    for name, exe in (("cpu", cpu), ("gpu", gpu)):
+        d = tmp_path / name
+        (d / "data").mkdir(parents=True)
+        print(f"reference program {ex} ({name})", flush=True)
+        r = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, timeout=240, preexec_fn=_big_stack)
+        assert r.returncode == 0 and "== end ==" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+        runs[name] = (r.stdout, sorted(p.name for p in (d / "data").iterdir()))
+    assert runs["cpu"][0] == runs["gpu"][0]
+    files = runs["cpu"][1]
+    assert files == runs["gpu"][1] and len(files) >= 3
+    for f in files:
+        assert filecmp.cmp(tmp_path / "cpu" / "data" / f, tmp_path / "gpu" / "data" / f, shallow=False), f
