@@ -7,8 +7,10 @@
  *  1. Drop-in entry points, one per reference subroutine, with the reference's
  *     argument lists (Fortran column-major arrays including the single ghost
  *     layer, extents ix, jx[, kx]).  A Fortran caller reaches them through the
- *     iso_c_binding shims in openmhd_amd/fortran/omhd_shims.f90, which keep the
- *     reference names (u2v, limiter, flux_solver, ...), so a reference-style
+ *     iso_c_binding shims in openmhd_amd/fortran/ (omhd_shims.f90, 2D;
+ *     omhd_shims3d.f90, 3D; omhd_shims_kh.f90, omhd_shims_recon.f90 and
+ *     omhd_shims3d_recon.f90, the problem-specific routines), which keep the
+ *     reference names (u2v, limiter, flux_solver, ...), so a reference
  *     main.f90 links against them unchanged.  Array pointers may be host or
  *     device memory: host arrays are staged through device buffers per call,
  *     device arrays are used in place.  Every call returns when its results
@@ -71,12 +73,13 @@ int omhd_limiter(const double *wk, double *wL, double *wR, int32_t ix, int32_t j
                  int32_t type);
 
 /* flux_solver(F,VL,VR,ch,ix,jx[,kx],dir,type) -- 2D_basic/flux_solver.f90:1,
- * 3D_basic/flux_solver.f90:1.  type 3 (HLLD).  F is zeroed first, as in the
- * CPU build (flux_solver.f90:57). */
+ * 3D_basic/flux_solver.f90:1.  type 0 LLF, 1 HLL, 2 HLLC, 3 HLLD, 4 LHLLD
+ * (:27).  F is zeroed first, as in the CPU build (flux_solver.f90:57). */
 int omhd_flux_solver(double *F, const double *VL, const double *VR, double ch, int32_t ix, int32_t jx, int32_t kx,
                      int32_t dir, int32_t type);
 
-/* glm_ss2(U,ch,dt,ix,jx[,kx]) -- 2D_basic/glm_ss2.f90:1.  psi *= exp(-dt*ch/(2*0.2))
+/* glm_ss2(U,ch,dt,ix,jx[,kx]) -- 2D_basic/glm_ss2.f90:1, 3D_basic/glm_ss2.f90:1.
+ * psi *= exp(-dt*ch/(2*0.2))
  * over the whole array (ghosts included); the factor is computed on the host. */
 int omhd_glm_ss2(double *U, double ch, double dt, int32_t ix, int32_t jx, int32_t kx);
 
@@ -92,12 +95,14 @@ int omhd_set_dt(const double *U, const double *V, double *vmax, double *dt, doub
                 int32_t jx, int32_t kx, int32_t ndim);
 
 /* bc(U,ix,jx[,kx]) / bc_for_U(U,ix,jx[,kx]) -- 2D_basic/bc.f90:1,
- * 2D_KH/bc.f90:6, 3D_basic/bc.f90:1.  bc_type[6] per face (x-,x+,y-,y+,z-,z+),
+ * 2D_KH/bc.f90:6, 2D_reconnection/bc.f90:6, 3D_basic/bc.f90:1,
+ * 3D_reconnection_gpu/bc.cuf:7.  bc_type[6] per face (x-,x+,y-,y+,z-,z+),
  * sign[6*9] per face and slot (+1 / -1) for OMHD_BC_MIRROR faces. */
 int omhd_bc_u(double *U, int32_t ix, int32_t jx, int32_t kx, const int32_t *bc_type, const double *sign);
 
 /* Interface-state face patch along dir: the inline periodic patch of
- * 2D_basic/main.f90:101-102,119-120 and bc_for_F/G of 2D_KH/bc.f90:42,55. */
+ * 2D_basic/main.f90:101-102,119-120 and bc_for_F/G(/H) of 2D_KH/bc.f90:42,55,
+ * 2D_reconnection/bc.f90:54,86 and 3D_reconnection_gpu/bc.cuf:103,147,191. */
 int omhd_bc_face(double *VL, double *VR, int32_t ix, int32_t jx, int32_t kx, int32_t dir, int32_t bc_type,
                  const double *sign_lo, const double *sign_hi);
 
