@@ -103,7 +103,7 @@ def cpu_baseline(n: int, sample_steps: int):
             return {"value": v, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
                     "single_thread": v1, "cpu": cpu_model(),
                     "sample": f"KH {n}x{n} minmod, steps 2..{sample_steps + 1} of a {sample_steps + 1}-step run "
-                              f"(driver's own KH layer init), the reference's routines compiled unmodified with "
+                              f"(the Fortran driver's own KH layer init), the reference's routines compiled unmodified with "
                               f"flang -O2 -fopenmp, {threads} OpenMP threads; single_thread: step 2 of a 2-step "
                               f"run on 1 thread"}
     # C restatement (port)

@@ -58,7 +58,7 @@ def orszag_tang(n: int):
     """Orszag-Tang vortex on [0,2pi]^2 with n x n cells, as 2D_basic/model.f90:
     rho = gamma^2, p = gamma, v = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0),
     psi = 0.  Returns (U, dx); ghost cells hold the analytic values (the
-    driver's first bc() overwrites them)."""
+    caller's first bc() overwrites them)."""
     ix = jx = n + 2
     pi = 4.0 * np.arctan(1.0)
     dx = (2 * pi - 0.0) / float(ix - 2)  # model.f90:19

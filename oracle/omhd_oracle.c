@@ -1,7 +1,7 @@
 /*
  * omhd_oracle.c -- CPU restatement of OpenMHD's finite-volume step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * TEST INFRASTRUCTURE ONLY.  This file is the test oracle for the HIP library in
  * openmhd_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline leg
  * of bench.py may load it.  Nothing in the product links or calls it.
  *
