@@ -83,7 +83,7 @@ def cpu_baseline(n: int, sample_steps: int):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
 
     def big_stack():  # set_dt/glm_ss2 keep ix*jx automatic arrays on the stack
-        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.getrlimit(resource.RLIMIT_STACK)[1],) * 2)
 
     def ref_rate(nthreads, steps):
         cmd = [str(drv), "--kh", str(n), str(steps + 1), "--time"]

@@ -182,7 +182,7 @@ REFPROG = REPO / "oracle" / "_ref"
 def _big_stack():  # set_dt/glm_ss2 keep automatic arrays on the stack
     import resource
 
-    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.getrlimit(resource.RLIMIT_STACK)[1],) * 2)
 
 
 @pytest.mark.parametrize("ex", ("1d_basic", "2d_basic"))
