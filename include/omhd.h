@@ -207,9 +207,10 @@ int omhd_get_timing(omhd_handle *h, double *stage_ms, int64_t *launches);
  * advanced together by one persistent launch per RK stage over a tile table.
  * Members share lm_type and time_type.  In host dt mode (default) each
  * member's dt and GLM factor are computed on the host exactly as omhd_step
- * computes them, so every member is bitwise equal to its solo run; in device
- * dt mode they are computed on the GPU (device exp) and whole steps can be
- * captured in a hipGraph (use_graph). */
+ * computes them; in device dt mode they are computed on the GPU, the GLM
+ * factor looked up in a per-member table of the host's exp values, and whole
+ * steps can be captured in a hipGraph (use_graph).  In both modes every
+ * member is bitwise equal to its solo run. */
 typedef struct omhd_batch omhd_batch;
 int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch **b);
 int omhd_batch_upload(omhd_batch *b, int32_t member, const double *U);

@@ -484,12 +484,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   }
 }
 
+// GLM factors of the device-dt mode.  With dt = (cfl*dx)/ch from the CFL
+// speed ch, the argument x = ((-0.5*dt)*ch)/0.2 of glm_ss2.f90:27 lies within
+// a few ulps of x0 = (-0.5*(cfl*dx))/0.2, since fl(dt*ch) is within two ulps
+// of cfl*dx and the scaling by -0.5 is exact.  The host tabulates its libm's
+// exp at the GLM_TAB_HALF doubles either side of x0 per member, and the
+// device looks the factor up by the bit offset of its x from x0: the same
+// bits as the host-dt path.
+constexpr int GLM_TAB_HALF = 64, GLM_TAB = 2 * GLM_TAB_HALF + 1;
+
 // Device dt: for each of n members, dt and the GLM factor from its reduced
 // CFL speed (set_dt.f90:52, glm_ss2.f90:27, flux_resistive.cuf:8), then the
-// speed slot is cleared for the next step's reduction.  exp is the device
-// libm's, so a run in this mode is compared with runs in the same mode.
+// speed slot is cleared for the next step's reduction.  The GLM factor comes
+// from the member's table of host exp values (glm_tab, glm_x0: GLM_TAB
+// doubles and the bits of x0 per member); an argument outside the table,
+// which a CFL-limited dt cannot produce, falls back to the device's exp.
 __global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const double *dx, const double *cfl,
-                            const int *tt, const double *eta_max, int *err, int n) {
+                            const int *tt, const double *eta_max, const double *glm_tab, const long long *glm_x0,
+                            int *err, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double vmax = __longlong_as_double((long long)vmax_bits[i]);
@@ -507,7 +519,11 @@ __global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const do
   q.ch2 = vmax * vmax;
   q.dtx1 = tt[i] == 0 ? dt / dx[i] : 0.5 * dt / dx[i];
   q.dtx2 = dt / dx[i];
-  q.glm_f = exp(-0.5 * dt * vmax / 0.2);
+  const double x = -0.5 * dt * vmax / 0.2;
+  const long long off = (long long)__double_as_longlong(x) - glm_x0[i];
+  q.glm_f = (off >= -GLM_TAB_HALF && off <= GLM_TAB_HALF)
+                ? glm_tab[(size_t)i * GLM_TAB + (size_t)(off + GLM_TAB_HALF)]
+                : exp(x);
   q.dt = dt;
   par[i] = q;
   vmax_bits[i] = 0ull;
@@ -1172,6 +1188,8 @@ struct omhd_batch {
   DevPar *par_d = nullptr;
   double *dx_d = nullptr, *cfl_d = nullptr, *tsum_d = nullptr;
   int *tt_d = nullptr;
+  double *glm_tab_d = nullptr;     // n x GLM_TAB host exp values (device-dt GLM factors)
+  long long *glm_x0_d = nullptr;   // bits of each member's x0
   std::vector<DevPar> par_h;
   std::vector<unsigned long long> vmax_h;
   std::vector<double> vmax;  // CFL speed of each member's current state
@@ -1280,6 +1298,8 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
   if (e == hipSuccess) e = hipMalloc((void **)&b->cfl_d, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&b->tsum_d, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&b->tt_d, n * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->glm_tab_d, (size_t)n * GLM_TAB * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&b->glm_x0_d, n * sizeof(long long));
   if (e != hipSuccess) {
     omhd_batch_destroy(b);
     return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
@@ -1301,6 +1321,23 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
   OMHD_HIP(hipMemcpy(b->dx_d, dx.data(), n * sizeof(double), hipMemcpyHostToDevice));
   OMHD_HIP(hipMemcpy(b->cfl_d, cfl.data(), n * sizeof(double), hipMemcpyHostToDevice));
   OMHD_HIP(hipMemcpy(b->tt_d, tt.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  {  // the device-dt GLM factors: host libm exp around each member's x0
+    std::vector<double> tab((size_t)n * GLM_TAB);
+    std::vector<long long> x0b(n);
+    for (int i = 0; i < n; i++) {
+      const double a = cfl[i] * dx[i];
+      const double x0 = -0.5 * a / 0.2;
+      memcpy(&x0b[i], &x0, sizeof x0);
+      for (int j = -GLM_TAB_HALF; j <= GLM_TAB_HALF; j++) {
+        const long long xb = x0b[i] + j;
+        double x;
+        memcpy(&x, &xb, sizeof x);
+        tab[(size_t)i * GLM_TAB + (size_t)(j + GLM_TAB_HALF)] = exp(x);
+      }
+    }
+    OMHD_HIP(hipMemcpy(b->glm_tab_d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    OMHD_HIP(hipMemcpy(b->glm_x0_d, x0b.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+  }
   OMHD_HIP(hipMemset(b->err_d, 0, 8 * sizeof(int)));
   OMHD_HIP(hipMemset(b->tsum_d, 0, n * sizeof(double)));
   if (int rc = batch_tiles(b)) return rc;
@@ -1319,7 +1356,7 @@ extern "C" int omhd_batch_destroy(omhd_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (void *q : {(void *)b->arena, (void *)b->backup, (void *)b->mem_d, (void *)b->tiles_d, (void *)b->vmax_d,
                   (void *)b->err_d, (void *)b->par_d, (void *)b->dx_d, (void *)b->cfl_d, (void *)b->tsum_d,
-                  (void *)b->tt_d})
+                  (void *)b->tt_d, (void *)b->glm_tab_d, (void *)b->glm_x0_d})
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1402,7 +1439,7 @@ extern "C" int omhd_batch_set_mode(omhd_batch *b, int32_t device_dt, int32_t use
     OMHD_HIP(hipMemcpy(b->vmax_d, bits.data(), b->n * sizeof(unsigned long long), hipMemcpyHostToDevice));
     OMHD_HIP(hipMemcpy(b->tsum_d, b->t.data(), b->n * sizeof(double), hipMemcpyHostToDevice));
     k_device_dt<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->par_d, b->vmax_d, b->dx_d, b->cfl_d, b->tt_d, nullptr,
-                                                        b->err_d, b->n);
+                                                        b->glm_tab_d, b->glm_x0_d, b->err_d, b->n);
     OMHD_HIP(hipGetLastError());
     if (int rc = batch_check_err(b)) return rc;
   }
@@ -1439,7 +1476,7 @@ static int batch_device_step(omhd_batch *b) {
   k_accumulate_t<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->tsum_d, b->par_d, b->n);
   if (int rc = batch_launch_stages(b)) return rc;
   k_device_dt<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->par_d, b->vmax_d, b->dx_d, b->cfl_d, b->tt_d, nullptr,
-                                                      b->err_d, b->n);
+                                                      b->glm_tab_d, b->glm_x0_d, b->err_d, b->n);
   OMHD_HIP(hipGetLastError());
   return OMHD_OK;
 }

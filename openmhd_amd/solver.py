@@ -176,11 +176,11 @@ class Ensemble:
     """n independent 2D grids advanced together (omhd_batch_*): one persistent
     launch per RK stage over a tile table of all members, each member with its
     own CFL dt.  configs: a list of omhd_config (make_config(...)); members share
-    the limiter and time type.  device_dt=False: every member is bitwise equal
-    to its solo Solver run; device_dt=True (+ graph=True): dt and the GLM
-    factor computed on the GPU, whole steps captured in a hipGraph (the
-    device's exp can differ from the host libm's in the last bit, so members
-    then match their solo runs to about 1e-10 rather than bitwise)."""
+    the limiter and time type.  device_dt=False: dt and the GLM factor from
+    the host each step; device_dt=True (+ graph=True): computed on the GPU (the
+    GLM factor from a per-member table of the host's exp values), whole steps
+    captured in a hipGraph.  Either way every member is bitwise equal to its
+    solo Solver run."""
 
     def __init__(self, configs):
         self.n = len(configs)

@@ -67,6 +67,10 @@ def test_autotune_keeps_state_and_results():
 
 
 def test_device_dt_graph_replay():
+    """Device dt with whole steps replayed from a hipGraph: every member, its
+    time and its state, bitwise equal to its solo run (host dt, host libm exp
+    for the GLM factor, which the device looks up in the member's table) and to
+    the same member run alone in the same mode without the graph."""
     from openmhd_amd import Ensemble
 
     cfgs, states = members(n=6, seed=5)
@@ -78,35 +82,48 @@ def test_device_dt_graph_replay():
         e.step(3)
         _ = e.times()
     for i, (c, U) in enumerate(zip(cfgs, states)):
-        one = Ensemble([c])  # the member's solo run in the same dt mode, eagerly
+        one = Ensemble([c])  # the member alone in the same dt mode, eagerly
         one.upload(0, U)
         one.set_mode(device_dt=True, graph=False)
         one.step(12)
         assert bits_equal(e.download(i), one.download(0)), i
         assert bits_equal(e.times()[i], one.times()[0]), i
         one.close()
-        _, Us = solo(c, U, 12)  # host-dt solo run: same up to the device exp
-        Ue = e.download(i)
-        rel = np.max(np.abs(Ue - Us)) / np.max(np.abs(Us))
-        assert rel < 1e-10, (i, rel)
+        dts, Us = solo(c, U, 12)  # the plain solo run
+        assert bits_equal(e.download(i), Us), i
+        assert bits_equal(e.times()[i], sum_in_order(dts)), i
     e.close()
 
 
+def sum_in_order(v):
+    t = 0.0
+    for x in v:
+        t = t + float(x)
+    return t
+
+
 def test_benchmark_ensemble_members_equal_solo_runs():
-    """The 256-member KH ensemble of the benchmark (sides 64-1024), host dt,
-    autotuned launch: 3 steps, every member's dt sequence and state bitwise
-    equal to its solo run."""
+    """The 256-member KH ensemble of the benchmark (sides 64-1024), autotuned
+    launch, 3 steps, in both dt modes (host dt; device dt with hipGraph
+    replay): every member's state bitwise equal to its solo run, and in host
+    dt mode its dt sequence too."""
     from openmhd_amd import Ensemble
 
     cfgs, states = members(n=256, seed=2026, smin=64, smax=1024)
     print(f"ensemble of {len(cfgs)} members built", flush=True)
-    e = Ensemble(cfgs)
-    for i, U in enumerate(states):
-        e.upload(i, U)
-    e.autotune(2)
-    tr = e.step(3, trace=True)
-    for i, (c, U) in enumerate(zip(cfgs, states)):
-        dts, Us = solo(c, U, 3)
-        assert bits_equal(tr[:, i], dts), i
-        assert bits_equal(e.download(i), Us), i
-    e.close()
+    ref = [solo(c, U, 3) for c, U in zip(cfgs, states)]
+    for device_dt in (False, True):
+        print(f"ensemble device_dt={device_dt}", flush=True)
+        e = Ensemble(cfgs)
+        for i, U in enumerate(states):
+            e.upload(i, U)
+        if device_dt:
+            e.set_mode(device_dt=True, graph=True)
+        e.autotune(2)
+        tr = e.step(3, trace=not device_dt)
+        for i, (dts, Us) in enumerate(ref):
+            if not device_dt:
+                assert bits_equal(tr[:, i], dts), i
+            assert bits_equal(e.download(i), Us), (device_dt, i)
+            assert bits_equal(e.times()[i], sum_in_order(dts)), (device_dt, i)
+        e.close()
