@@ -67,6 +67,29 @@ __device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k) {
   return true;
 }
 
+// The same bijection of threads onto the grid's cells as cell_of, visited in
+// another order, for a kernel whose 5-point stencils along y and z re-read
+// neighbouring rows: blocks are dealt to the 8 XCDs in contiguous ranges (the
+// dispatcher hands workgroup b to XCD b % 8, and each XCD has its own L2), and
+// the rows (j, k) are walked in tiles of TY x TZ rows, TY and TZ the largest
+// powers of two up to 16 and 8 that divide ny and nz, so that the rows in
+// flight on one XCD are each other's neighbours along both axes.
+__device__ __forceinline__ bool cell_of_tiled(const Geo &g, int &i, int &j, int &k) {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned L = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
+  const long long t = (long long)L * blockDim.x + threadIdx.x;
+  if (t >= (long long)g.nx * g.ny * g.nz) return false;
+  i = (int)(t % g.nx);
+  const long long q = t / g.nx;  // row ordinal in visiting order, < ny * nz
+  const int TY = min(16, g.ny & -g.ny), TZ = min(8, g.nz & -g.nz);
+  const int in = (int)(q % (TY * TZ));
+  const long long T = q / (TY * TZ);
+  const int nty = g.ny / TY;
+  j = (int)(T % nty) * TY + in % TY;
+  k = (int)(T / nty) * TZ + in / TY;
+  return true;
+}
+
 __device__ __forceinline__ int map_ax(int c, int n, int bc, unsigned slo, unsigned shi, unsigned &mask) {
   if (c < 0) {
     if (bc == OMHD_BC_PERIODIC) return c + n;
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *_
                                                     const double *__restrict__ AL, const double *__restrict__ AR,
                                                     long long astride, const double *__restrict__ R, const Combine c) {
   int i, j, k;
-  if (!cell_of(g, i, j, k)) return;
+  if (!cell_of_tiled(g, i, j, k)) return;
   const long long o0 = (long long)k * g.zstride + (long long)j * g.pitch + i;
   double b[NV], wc[NV];
 #pragma unroll
