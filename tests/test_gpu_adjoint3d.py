@@ -64,6 +64,10 @@ CASES = [
     (8, 6, 5, ("periodic",) * 3, 2, 0, 0, 3, 2, None),
     (8, 6, 5, ("periodic", "wall", "periodic"), 1, 0, 1, 3, 2, None),
     (8, 6, 6, ("wall", "wall", "wall"), 2, 1, 1, 2, 1, None),
+    # large enough for the gather's visiting order to deal whole block ranges
+    # to the XCDs (a block count divisible by 8) and, in 3D, 16 x 8 row tiles
+    (64, 32, 0, ("periodic", "wall", "periodic"), 2, 0, 0, 2, 1, None),
+    (32, 16, 16, ("periodic", "wall", "periodic"), 1, 0, 1, 2, 1, None),
 ]
 
 
