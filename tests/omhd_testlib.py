@@ -9,12 +9,17 @@ e.g. U[ix, jx, 9] (2D) or U[ix, jx, kx, 9] (3D); slot order follows param.h:6-15
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 REPO = Path(__file__).resolve().parents[1]
-ORACLE_SO = REPO / "oracle" / "liboracle.so"
+# OMHD_SANITIZED=1: the AddressSanitizer + UndefinedBehaviorSanitizer builds of
+# the oracle and of the native checkers (oracle/run_sanitized.sh)
+SANITIZED = os.environ.get("OMHD_SANITIZED") == "1"
+ORACLE_SO = REPO / "oracle" / ("_san/liboracle.so" if SANITIZED else "liboracle.so")
+NATIVE_DIR = REPO / "tests" / "native" / ("_san" if SANITIZED else "")
 REF_DIR = REPO / "oracle" / "_ref"
 
 GAMMA = 5.0 / 3.0
@@ -512,9 +517,9 @@ class AdjCheck:
         import subprocess
 
         d = Path(__file__).resolve().parent / "native"
-        so = d / "libadjcheck.so"
+        so = NATIVE_DIR / "libadjcheck.so"
         if not so.exists():
-            subprocess.run(["make", "-s", "-C", str(d)], check=True)
+            subprocess.run(["make", "-s", "-C", str(d)] + (["sanitize"] if SANITIZED else []), check=True)
         self.L = C.CDLL(str(so))
 
     def __getattr__(self, name):

@@ -8,7 +8,9 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-LIB = Path(__file__).resolve().parent / "native" / "libflopcount.so"
+from omhd_testlib import NATIVE_DIR
+
+LIB = NATIVE_DIR / "libflopcount.so"
 LP = C.POINTER(C.c_long)
 
 
