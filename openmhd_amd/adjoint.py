@@ -13,6 +13,12 @@ counterpart in the reference, which has no derivatives.
     WT = roll(W0)                                      # W0: (9, ny, nx) cuda f64, requires_grad
     loss = ((WT[6] - target) ** 2).sum()
     loss.backward()                                    # W0.grad via omhd_adjoint_backward
+
+A resistive solver (Solver(..., resistivity=(eta0, eta1, w))) also takes the
+resistivity parameters as a tensor, so they can be fitted the same way:
+
+    eta = torch.tensor([1e-3, 1e-2, 2.0], dtype=torch.float64, requires_grad=True)
+    WT = roll(W0, eta=eta)                             # eta.grad via omhd_adjoint_eta_grad
 """
 from __future__ import annotations
 
@@ -82,38 +88,49 @@ def _function():
 
     class RolloutFunction(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, W0, roll, dt, ch):
-            torch.cuda.current_stream(W0.device).synchronize()  # W0 is complete before libomhd reads it
+        def forward(ctx, W0, eta, roll, dt, ch):
             s = roll.solver
-            s.set_state(W0.detach().contiguous())
+            if eta is not None:  # (eta0, eta1, w) of the localized resistivity
+                s.set_resistivity(*(float(v) for v in eta.detach().cpu().reshape(3)))
+            s.set_state(W0.detach().contiguous())  # waits for torch's stream (the copy included)
             adj = Adjoint(s, roll.nsteps, roll.every)
             roll.dt, roll.ch = adj.forward(dt, ch)
             WT = torch.empty(W0.shape, dtype=W0.dtype, device=W0.device)  # dense, whatever W0's strides
             s.get_state(WT)  # synchronises the library's stream
             ctx.adj = adj
+            ctx.eta_like = None if eta is None else (eta.dtype, eta.device, eta.shape)
             return WT
 
         @staticmethod
         def backward(ctx, gWT):
             g = gWT.contiguous()
-            torch.cuda.current_stream(g.device).synchronize()
             out = torch.empty_like(g)
-            ctx.adj.backward(g, out)  # synchronises the library's stream
-            return out, None, None, None
+            ctx.adj.backward(g, out)  # waits for torch's stream, returns with the library's drained
+            geta = None
+            if ctx.eta_like is not None:  # dL/d(eta0, eta1, w) of this backward pass
+                dtype, device, shape = ctx.eta_like
+                geta = torch.tensor(ctx.adj.eta_grad(), dtype=dtype, device=device).reshape(shape)
+            return out, geta, None, None, None
 
     return RolloutFunction
 
 
 class Rollout:
     """nsteps steps of `solver` as a differentiable function of the initial
-    state W0 (a (9, ny, nx) float64 CUDA tensor).  After a call, .dt and .ch
-    hold the step sequence used; pass dt=, ch= to impose one (e.g. to repeat
-    a run with the same frozen sequence, as finite-difference checks do)."""
+    state W0 (a (9, ny, nx) or (9, nz, ny, nx) float64 CUDA tensor) and, for
+    a resistive solver, of eta = (eta0, eta1, w), the parameters of
+    eta0 + eta1 exp(-(x^2 + y^2)/w^2) (a float64 tensor of 3 values; the
+    solver keeps them after the call).  After a call, .dt and .ch hold the
+    step sequence used; pass dt=, ch= to impose one (e.g. to repeat a run with
+    the same frozen sequence, as finite-difference checks do).  The sequence
+    is not differentiated, including the resistive limit on dt."""
 
     def __init__(self, solver, nsteps: int, every: int = 20):
         self.solver, self.nsteps, self.every = solver, int(nsteps), int(every)
         self.dt = self.ch = None
         self._fn = _function()
 
-    def __call__(self, W0, dt=None, ch=None):
-        return self._fn.apply(W0, self, dt, ch)
+    def __call__(self, W0, dt=None, ch=None, eta=None):
+        if eta is not None and eta.numel() != 3:
+            raise ValueError("eta: the three values (eta0, eta1, w)")
+        return self._fn.apply(W0, eta, self, dt, ch)

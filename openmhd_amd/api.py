@@ -49,6 +49,18 @@ def _fortran_strides(shape, strides) -> bool:
     return True
 
 
+def torch_ready(t) -> None:
+    """Wait for the work torch has queued on `t`'s device.  libomhd runs on
+    streams of its own that do not wait for torch's, so a tensor that a torch
+    kernel is still writing must be complete before its pointer is handed over
+    (every libomhd entry point returns with its own stream drained, so the
+    other direction needs nothing)."""
+    if t.is_cuda:
+        import torch
+
+        torch.cuda.current_stream(t.device).synchronize()
+
+
 def _ptr(a, n_expected: int | None = None):
     """Device or host pointer of a float64 array laid out in Fortran order."""
     if a is None:
@@ -62,6 +74,7 @@ def _ptr(a, n_expected: int | None = None):
             raise ValueError("Fortran-ordered (column-major) contiguous tensor required")
         if n_expected is not None and a.numel() < n_expected:
             raise ValueError(f"tensor has {a.numel()} elements, expected {n_expected}")
+        torch_ready(a)
         return C.cast(C.c_void_p(a.data_ptr()), DP)
     if not isinstance(a, np.ndarray) or a.dtype != np.float64:
         raise TypeError("float64 numpy array required")

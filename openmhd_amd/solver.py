@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import DP, I32, OMHD_BC_MIRROR, OMHD_BC_PERIODIC, Config, OMHDError, check, lib
-from .api import KH_WALL_SIGN, _ptr
+from .api import KH_WALL_SIGN, _ptr, torch_ready
 
 
 def _cptr(a, n_expected):
@@ -22,6 +22,7 @@ def _cptr(a, n_expected):
             raise TypeError("contiguous float64 tensor required")
         if a.numel() != n_expected:
             raise ValueError(f"tensor has {a.numel()} elements, expected {n_expected}")
+        torch_ready(a)
         return C.cast(C.c_void_p(a.data_ptr()), DP)
     if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("C-contiguous float64 numpy array required")

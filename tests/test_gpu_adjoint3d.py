@@ -159,3 +159,58 @@ def test_eta_parameter_gradient(ac, oracle):
     assert max(best) <= 1e-6, best
     adj.close()
     s.close()
+
+
+def test_rollout_takes_eta_as_a_tensor(ac, oracle):
+    """Rollout(W0, eta=tensor): the resistivity parameters are a torch input.
+    At parameters other than the solver's own, torch.autograd gives eta.grad
+    equal bit for bit to Adjoint.eta_grad() (and W0.grad to Adjoint.backward)
+    of the same run, and equal to the exact tangent along d eta/d eta1 and
+    d eta/d w to 1e-10."""
+    import torch
+
+    from openmhd_amd import Adjoint, Rollout, snapshot
+
+    nx, ny, nz, nsteps = 10, 8, 6, 4
+    bc = ("periodic", "wall", "periodic")
+    s, U, p, W0, _, bca, sg, (e0, e1, w), origin = setup(oracle, nx, ny, nz, bc, 2, 0, 1, 11)
+    prm = (1.5 * e0, 0.75 * e1, 0.625)  # not the solver's creation values
+    lamT = np.random.default_rng(5).standard_normal(W0.shape)
+    # the library's own calls
+    s.set_resistivity(*prm)
+    s.set_state(W0)
+    adj = Adjoint(s, nsteps, 2)
+    dts, chs = adj.forward()
+    gW = adj.backward(lamT)
+    g = adj.eta_grad()
+    adj.close()
+    # the same run through torch.autograd, starting from the creation values
+    s.set_resistivity(e0, e1, w)
+    W0t = torch.tensor(W0, device="cuda", requires_grad=True)
+    etat = torch.tensor(prm, dtype=torch.float64, requires_grad=True)
+    roll = Rollout(s, nsteps, 2)
+    WT = roll(W0t, eta=etat)
+    assert np.array_equal(roll.dt.view(np.int64), dts.view(np.int64))
+    (WT * torch.tensor(lamT, device="cuda")).sum().backward()
+    assert np.array_equal(etat.grad.numpy().view(np.int64), g.view(np.int64)), (etat.grad, g)
+    assert np.array_equal(W0t.grad.cpu().numpy().view(np.int64), gW.view(np.int64))
+    # exact tangents at prm
+    x = snapshot.grid(nx, 0.1, origin[0])
+    y = snapshot.grid(ny, 0.1, origin[1])
+    eta, _ = oracle.eta_faces(x, y, 0.1, *prm)
+    eta = np.ascontiguousarray(eta)
+    pe0, pe1, pw = prm
+    de1, dw = np.zeros((3, ny + 2, nx + 2)), np.zeros((3, ny + 2, nx + 2))
+    for d in range(3):
+        X, Y = np.meshgrid(x + (0.05 if d == 0 else 0.0), y + (0.05 if d == 1 else 0.0), indexing="xy")
+        r2 = X * X + Y * Y
+        de1[d] = np.exp(-r2 / (pw * pw))
+        dw[d] = pe1 * np.exp(-r2 / (pw * pw)) * 2 * r2 / pw ** 3
+    for k, de in ((1, de1), (2, dw)):
+        jv = np.zeros_like(W0)
+        ac.L.adjc3_run_jvp(nx, ny, nz, 2, 0, D(0.1), bca, cptr(sg), cptr(eta.reshape(-1)),
+                           cptr(np.ascontiguousarray(de.reshape(-1))), nsteps, cptr(dts), cptr(chs), cptr(W0),
+                           cptr(np.zeros_like(W0)), cptr(jv))
+        tl = float((lamT * jv).sum())
+        assert abs(etat.grad[k].item() - tl) <= 1e-10 * abs(tl), (k, etat.grad[k].item(), tl)
+    s.close()
