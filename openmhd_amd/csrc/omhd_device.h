@@ -32,6 +32,30 @@ __device__ __forceinline__ double fsign(double a, double b) { return __builtin_c
 __device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
 
+// Square root with the bits of __builtin_sqrt in fewer instructions.  hipcc
+// expands an f64 sqrt for gfx950 into: a scale of x by 2^256 when x < 2^-767,
+// v_rsq_f64, one coupled Newton step for g ~ sqrt(x) and h ~ 1/(2 sqrt(x)),
+// two corrections g += (x - g*g)*h, the inverse scale by 2^-128, and a select
+// that returns x itself for +-0 and +inf.  On [2^-767, DBL_MAX] the scale
+// factors are 2^0 and the select is not taken, so there the core below (the
+// same operations on the same operands, in the same order) gives the same
+// result with 8 fewer instructions.  Every other argument (zero, denormals,
+// inf, NaN, negatives) goes to __builtin_sqrt.
+__device__ __forceinline__ double csqrt(double x) {
+  if (__builtin_expect(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023, 1)) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  }
+  return __builtin_sqrt(x);
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level lane shifts (64-wide wavefront, DPP wave_shr/wave_shl by 1).
 // from_prev: lane l receives lane l-1 (lane 0 receives 0).
@@ -140,7 +164,7 @@ __device__ __forceinline__ void hlld(const double L[NV], const double R[NV], dou
   FL[8] = 0.0;
   double f1 = GAM * L[PR];
   double f2 = 4 * (L[bn] * L[bn]);
-  const double cfL = ((f1 + B2L) + __builtin_sqrt(dmax((f1 + B2L) * (f1 + B2L) - f1 * f2, 0.0))) / (2 * L[RO]);
+  const double cfL = ((f1 + B2L) + csqrt(dmax((f1 + B2L) * (f1 + B2L) - f1 * f2, 0.0))) / (2 * L[RO]);
   const double ptL = L[PR] + 0.5 * B2L;
 
   // right
@@ -163,11 +187,11 @@ __device__ __forceinline__ void hlld(const double L[NV], const double R[NV], dou
   FR[8] = 0.0;
   f1 = GAM * R[PR];
   f2 = 4 * (R[bn] * R[bn]);
-  const double cfR = ((f1 + B2R) + __builtin_sqrt(dmax((f1 + B2R) * (f1 + B2R) - f1 * f2, 0.0))) / (2 * R[RO]);
+  const double cfR = ((f1 + B2R) + csqrt(dmax((f1 + B2R) * (f1 + B2R) - f1 * f2, 0.0))) / (2 * R[RO]);
   const double ptR = R[PR] + 0.5 * B2R;
 
   // Riemann fan (:408-410): one shared fast speed
-  f1 = __builtin_sqrt(dmax(cfL, cfR));
+  f1 = csqrt(dmax(cfL, cfR));
   const double aL = dmin(L[vn], R[vn]) - f1;
   const double aR = dmax(L[vn], R[vn]) + f1;
 
@@ -189,7 +213,7 @@ __device__ __forceinline__ void hlld(const double L[NV], const double R[NV], dou
     const double pt = ptL + L[RO] * (aL - L[vn]) * (aM - L[vn]);    // :431
     const double roL = L[RO] * (aL - L[vn]) / (aL - aM);            // :432
     const double roR = R[RO] * (aR - R[vn]) / (aR - aM);            // :433
-    const double sroL = __builtin_sqrt(roL), sroR = __builtin_sqrt(roR);
+    const double sroL = csqrt(roL), sroR = csqrt(roR);
     const double aVL = __builtin_fabs(Uh[bn]) / sroL;              // :436
     const double aVR = __builtin_fabs(Uh[bn]) / sroR;              // :437
     const double hb = Uh[bn];
@@ -328,7 +352,7 @@ __device__ __forceinline__ void side_state(const double S[NV], double U[4], doub
 // of gamma p it is LHLLD's second estimate
 __device__ __forceinline__ double fast2(double f1, double B2, double bnv, double ro) {
   const double f2 = 4 * (bnv * bnv);
-  return ((f1 + B2) + __builtin_sqrt(dmax((f1 + B2) * (f1 + B2) - f1 * f2, 0.0))) / (2 * ro);
+  return ((f1 + B2) + csqrt(dmax((f1 + B2) * (f1 + B2) - f1 * f2, 0.0))) / (2 * ro);
 }
 
 template <int D>
@@ -344,10 +368,10 @@ __device__ __forceinline__ void riemann_other(int type, const double L[NV], cons
   for (int m = 0; m < NV; m++) F[m] = 0.0;
   if (type == 0) {  // LLF: no guard under the inner sqrt (:112-113)
     double f1 = GAM * L[PR], f2 = 4 * (L[bn] * L[bn]);
-    const double aL = __builtin_sqrt(((f1 + B2L) + __builtin_sqrt((f1 + B2L) * (f1 + B2L) - f1 * f2)) / (2 * L[RO]));
+    const double aL = csqrt(((f1 + B2L) + csqrt((f1 + B2L) * (f1 + B2L) - f1 * f2)) / (2 * L[RO]));
     f1 = GAM * R[PR];
     f2 = 4 * (R[bn] * R[bn]);
-    const double aR = __builtin_sqrt(((f1 + B2R) + __builtin_sqrt((f1 + B2R) * (f1 + B2R) - f1 * f2)) / (2 * R[RO]));
+    const double aR = csqrt(((f1 + B2R) + csqrt((f1 + B2R) * (f1 + B2R) - f1 * f2)) / (2 * R[RO]));
     f1 = dmax(__builtin_fabs(L[vn]) + aL, __builtin_fabs(R[vn]) + aR);
 #pragma unroll
     for (int m = 0; m < 4; m++) F[m] = 0.5 * (FL[m] + FR[m] - f1 * (UR[m] - UL[m]));
@@ -357,7 +381,7 @@ __device__ __forceinline__ void riemann_other(int type, const double L[NV], cons
   } else if (type == 1 || type == 2) {  // HLL, HLLC: fan speeds clamped at 0 (:207-208)
     double aL = fast2(GAM * L[PR], B2L, L[bn], L[RO]), aR = fast2(GAM * R[PR], B2R, R[bn], R[RO]);
     const double ptL = L[PR] + 0.5 * B2L, ptR = R[PR] + 0.5 * B2R;
-    double f1 = __builtin_sqrt(dmax(aL, aR));
+    double f1 = csqrt(dmax(aL, aR));
     aL = dmin(dmin(L[vn], R[vn]) - f1, 0.0);
     aR = dmax(dmax(L[vn], R[vn]) + f1, 0.0);
     f1 = 1.0 / (aR - aL);
@@ -396,8 +420,8 @@ __device__ __forceinline__ void riemann_other(int type, const double L[NV], cons
     const double ptL = L[PR] + 0.5 * B2L, ptR = R[PR] + 0.5 * B2R;
     double aL1 = fast2(L[RO] * v2L, B2L, L[bn], L[RO]), aR1 = fast2(R[RO] * v2R, B2R, R[bn], R[RO]);
     double f1 = dmin(1.0, dmax(aL1, aR1) / dmax(aL, aR));  // chi^2
-    const double psi = 2 * __builtin_sqrt(f1) - f1;
-    f1 = __builtin_sqrt(dmax(aL, aR));
+    const double psi = 2 * csqrt(f1) - f1;
+    f1 = csqrt(dmax(aL, aR));
     aL = dmin(L[vn], R[vn]) - f1;
     aR = dmax(L[vn], R[vn]) + f1;
     if (aL >= 0) {
@@ -419,7 +443,7 @@ __device__ __forceinline__ void riemann_other(int type, const double L[NV], cons
                         (R[RO] * (aR - R[vn]) - L[RO] * (aL - L[vn]));
       const double roL = L[RO] * (aL - L[vn]) / (aL - aM);
       const double roR = R[RO] * (aR - R[vn]) / (aR - aM);
-      const double sroL = __builtin_sqrt(roL), sroR = __builtin_sqrt(roR);
+      const double sroL = csqrt(roL), sroR = csqrt(roR);
       const double aVL = __builtin_fabs(Uh[bn]) / sroL;
       const double aVR = __builtin_fabs(Uh[bn]) / sroR;
       const double hb = Uh[bn], hb2 = hb * hb;
@@ -510,13 +534,13 @@ __device__ __forceinline__ double fastest_speed(const double u[NV], const double
   const double ro2 = 2 * u[4];
   const double s = f1 + B2;
   double f2 = 4 * f1 * (u[5] * u[5]);
-  const double vfx = __builtin_sqrt((s + __builtin_sqrt(dmax(s * s - f2, 0.0))) / ro2);
+  const double vfx = csqrt((s + csqrt(dmax(s * s - f2, 0.0))) / ro2);
   f2 = 4 * f1 * (u[6] * u[6]);
-  const double vfy = __builtin_sqrt((s + __builtin_sqrt(dmax(s * s - f2, 0.0))) / ro2);
+  const double vfy = csqrt((s + csqrt(dmax(s * s - f2, 0.0))) / ro2);
   double vt = dmax(__builtin_fabs(w[0]) + vfx, __builtin_fabs(w[1]) + vfy);
   if constexpr (ND == 3) {
     f2 = 4 * f1 * (u[7] * u[7]);
-    const double vfz = __builtin_sqrt((s + __builtin_sqrt(dmax(s * s - f2, 0.0))) / ro2);
+    const double vfz = csqrt((s + csqrt(dmax(s * s - f2, 0.0))) / ro2);
     vt = dmax(vt, __builtin_fabs(w[2]) + vfz);
   }
   return vt;

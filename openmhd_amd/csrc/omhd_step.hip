@@ -124,6 +124,18 @@ struct SweepLdsRes {
   double bxm[66];        // XY: Bx of row rn-3
 };
 
+// Element of variable m at this lane's column of a row: `ro` is the row's
+// offset (uniform across the wavefront) and off8 the lane's byte offset in
+// the row.  The uniform part stays in scalar registers; the compiler adds the
+// lane's offset once per row and variable.
+__device__ __forceinline__ const double &lane_at(const double *a, long long plane, int m, long long ro,
+                                                 unsigned off8) {
+  return *(const double *)((const char *)(a + (m * plane + ro)) + off8);
+}
+__device__ __forceinline__ double &lane_at(double *a, long long plane, int m, long long ro, unsigned off8) {
+  return *(double *)((char *)(a + (m * plane + ro)) + off8);
+}
+
 __device__ __forceinline__ void put66(double *row, int lane, double v) {
   row[lane + 1] = v;
   if (lane == 0) row[0] = v;
@@ -175,19 +187,21 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
   double Gl[NV], nxt[NV];
   double vloc = 0.0;
 
-  auto row_addr = [&](int rn, unsigned &mask) -> long long {
+  const unsigned cm8 = (unsigned)cm * 8u;  // this lane's byte offset in a row
+  // offset of the first cell of (mapped) row rn, uniform across the wavefront
+  auto row_off = [&](int rn, unsigned &mask) -> long long {
     int rm;
     unsigned ymask;
     map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ymask);
     mask = xmask ^ ymask;
-    return fixoff + (long long)rm * rstride + cm;
+    return fixoff + (long long)rm * rstride;
   };
 
   unsigned mask_nxt;
   {
-    const long long o = row_addr(j0 - 2, mask_nxt);
+    const long long ro = row_off(j0 - 2, mask_nxt);
 #pragma unroll
-    for (int m = 0; m < NV; m++) nxt[m] = p.us[m * plane + o];
+    for (int m = 0; m < NV; m++) nxt[m] = lane_at(p.us, plane, m, ro, cm8);
   }
 
   for (int k = 0; k < hc + 4; ++k) {
@@ -195,14 +209,19 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     const int sa = (k + 1) % 3, sb = (k + 2) % 3, sc = k % 3;  // rows rn-2, rn-1, rn
     {
       double u[NV], W[NV];
+      if (__any(mask_nxt != 0u)) {  // a ghost row or column with negated slots (mirror faces)
 #pragma unroll
-      for (int m = 0; m < NV; m++) u[m] = flip(nxt[m], mask_nxt, m);
+        for (int m = 0; m < NV; m++) u[m] = flip(nxt[m], mask_nxt, m);
+      } else {
+#pragma unroll
+        for (int m = 0; m < NV; m++) u[m] = nxt[m];
+      }
       u[8] = glm_us * u[8];  // glm_ss2 before the sweep (main.f90:87)
       if (k + 1 < hc + 4) {  // prefetch the next row
         unsigned mnext;
-        const long long o = row_addr(rn + 1, mnext);
+        const long long ro = row_off(rn + 1, mnext);
 #pragma unroll
-        for (int m = 0; m < NV; m++) nxt[m] = p.us[m * plane + o];
+        for (int m = 0; m < NV; m++) nxt[m] = lane_at(p.us, plane, m, ro, cm8);
         mask_nxt = mnext;
       }
       cons_to_prim(u, W);  // u2v
@@ -228,9 +247,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             unsigned zmask;
             map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zmask);
             const unsigned mk = xmask ^ ymask ^ zmask;
-            const long long o = (long long)km * p.zstride + (long long)rm * p.pitch + cm;
-            put66(lds.aux[sc][s2], lane, flip(p.us[5 * plane + o], mk, 5));
-            put66(lds.aux[sc][2 + s2], lane, flip(p.us[6 * plane + o], mk, 6));
+            const long long ro = (long long)km * p.zstride + (long long)rm * p.pitch;
+            put66(lds.aux[sc][s2], lane, flip(lane_at(p.us, plane, 5, ro, cm8), mk, 5));
+            put66(lds.aux[sc][2 + s2], lane, flip(lane_at(p.us, plane, 6, ro, cm8), mk, 6));
           }
         } else {
 #pragma unroll
@@ -239,8 +258,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             unsigned jmask;
             map_axis(slice + (s2 ? 1 : -1), p.ny, p.bc[1], p.slo[1], p.shi[1], jm, jmask);
             const unsigned mk = xmask ^ ymask ^ jmask;
-            const long long o = (long long)rm * p.zstride + (long long)jm * p.pitch + cm;
-            put66(lds.aux[sc][s2], lane, flip(p.us[7 * plane + o], mk, 7));
+            const long long ro = (long long)rm * p.zstride + (long long)jm * p.pitch;
+            put66(lds.aux[sc][s2], lane, flip(lane_at(p.us, plane, 7, ro, cm8), mk, 7));
           }
         }
       }
@@ -254,18 +273,18 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     // TVD stage 2 folds U^n + U* (the first sum of rk_tvd22) before the
     // march-axis flux, so the U* loads go out as one batch
     constexpr bool FOLD = FINAL && STAGE == 2 && TT == 0 && PREF;
+    // row rn-2, combined in this iteration: its offset (a writer lane's column c equals cm)
+    const long long ro2 = fixoff + (long long)(rn - 2) * rstride;
     if constexpr (FINAL && STAGE == 2 && PREF) {
       if (k >= 4 && writer) {
-        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
 #pragma unroll
-        for (int m = 0; m < NV; m++) a0p[m] = p.u0[m * plane + o];
+        for (int m = 0; m < NV; m++) a0p[m] = lane_at(p.u0, plane, m, ro2, cm8);
       }
     }
     if constexpr (FINAL && STAGE == 1 && PREF) {
       if (k >= 4 && writer) {
-        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
 #pragma unroll
-        for (int m = 0; m < NV; m++) usp[m] = p.us[m * plane + o];
+        for (int m = 0; m < NV; m++) usp[m] = lane_at(p.us, plane, m, ro2, cm8);
       }
     }
     // Z pass, stage 1, ideal MHD: the XY pass's partial flux difference for
@@ -274,9 +293,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     double azp[NV];
     if constexpr (ZPRE) {
       if (k >= 4 && writer) {
-        const long long o = fixoff + (long long)(rn - 2) * rstride + c;
 #pragma unroll
-        for (int m = 0; m < NV; m++) azp[m] = p.acc[m * plane + o];
+        for (int m = 0; m < NV; m++) azp[m] = lane_at((const double *)p.acc, plane, m, ro2, cm8);
       }
     }
 
@@ -285,11 +303,24 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     if constexpr (XS) {
       if (k >= 4) {
         double Lx[NV], Rx[NV], F[NV];
+        {
+          // all 27 neighbour reads in flight before the first slope: left to
+          // itself the scheduler issues them three at a time and waits on each
+          // group (one LDS latency per variable)
+          double x0[NV], x1[NV], x2[NV];
 #pragma unroll
-        for (int m = 0; m < NV; m++) {
-          double wRx;
-          limit<LIM>(lds.ring[sa][m][lane], lds.ring[sa][m][lane + 1], lds.ring[sa][m][lane + 2], wRx, Lx[m]);
-          Rx[m] = from_next(wRx);
+          for (int m = 0; m < NV; m++) {
+            x0[m] = lds.ring[sa][m][lane];
+            x1[m] = lds.ring[sa][m][lane + 1];
+            x2[m] = lds.ring[sa][m][lane + 2];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int m = 0; m < NV; m++) {
+            double wRx;
+            limit<LIM>(x0[m], x1[m], x2[m], wRx, Lx[m]);
+            Rx[m] = from_next(wRx);
+          }
         }
         if (xlo_face) {
 #pragma unroll
@@ -322,19 +353,28 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     if (k >= 2) {
       // march-axis slopes of row rn-1 from rows rn-2, rn-1, rn
       double Lf[NV], Rf[NV];
+      {
+        double y0[NV], y1[NV], y2[NV];  // likewise: every own-lane read before the first slope
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        double wLyB;
-        limit<LIM>(lds.ring[sa][m][lane + 1], lds.ring[sb][m][lane + 1], lds.ring[sc][m][lane + 1], Rf[m], wLyB);
-        Lf[m] = lds.wly[m][lane];  // left state of the interface above row rn-2
-        lds.wly[m][lane] = wLyB;
+        for (int m = 0; m < NV; m++) {
+          y0[m] = lds.ring[sa][m][lane + 1];
+          y1[m] = lds.ring[sb][m][lane + 1];
+          y2[m] = lds.ring[sc][m][lane + 1];
+          Lf[m] = lds.wly[m][lane];  // left state of the interface above row rn-2
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+          double wLyB;
+          limit<LIM>(y0[m], y1[m], y2[m], Rf[m], wLyB);
+          lds.wly[m][lane] = wLyB;
+        }
       }
       if constexpr (FOLD) {
         if (k >= 4 && writer) {
-          const long long o = fixoff + (long long)(rn - 2) * rstride + c;
           double usf[NV];
 #pragma unroll
-          for (int m = 0; m < NV; m++) usf[m] = p.us[m * plane + o];
+          for (int m = 0; m < NV; m++) usf[m] = lane_at(p.us, plane, m, ro2, cm8);
 #pragma unroll
           for (int m = 0; m < NV; m++) a0p[m] = (m == 8 ? glm_f * a0p[m] : a0p[m]) + usf[m];
         }
@@ -378,21 +418,19 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         }
 
         if (k >= 4 && writer) {
-          const int r = rn - 2;
-          const long long o = fixoff + (long long)r * rstride + c;
           double d[NV];
           if constexpr (PASS == PASS_2D) {
 #pragma unroll
             for (int m = 0; m < NV; m++) d[m] = tq[m] - G[m];
           } else if constexpr (PASS == PASS_XY) {
 #pragma unroll
-            for (int m = 0; m < NV; m++) p.acc[m * plane + o] = tq[m] - G[m];  // ((F-F)+G)-G
+            for (int m = 0; m < NV; m++) lane_at(p.acc, plane, m, ro2, cm8) = tq[m] - G[m];  // ((F-F)+G)-G
           } else {
 #pragma unroll
             for (int m = 0; m < NV; m++) {
               double a;
               if constexpr (ZPRE) a = azp[m];
-              else a = p.acc[m * plane + o];
+              else a = lane_at((const double *)p.acc, plane, m, ro2, cm8);
               d[m] = a + Gl[m] - G[m];  // (.. + H(k-1)) - H(k)
             }
           }
@@ -403,7 +441,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
               for (int m = 0; m < NV; m++) {
                 double uu;
                 if constexpr (PREF) uu = usp[m];
-                else uu = p.us[m * plane + o];
+                else uu = lane_at(p.us, plane, m, ro2, cm8);
                 if (m == 8) uu = glm_f * uu;
                 o9[m] = uu + dtx * d[m];  // rk_tvd21 / rk_std21
               }
@@ -415,16 +453,16 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
                 } else {
                   double a0;
                   if constexpr (PREF) a0 = a0p[m];
-                  else a0 = p.u0[m * plane + o];
+                  else a0 = lane_at(p.u0, plane, m, ro2, cm8);
                   if (m == 8) a0 = glm_f * a0;
-                  if constexpr (TT == 0) o9[m] = 0.5 * (a0 + p.us[m * plane + o] + dtx * d[m]);  // rk_tvd22
+                  if constexpr (TT == 0) o9[m] = 0.5 * (a0 + lane_at(p.us, plane, m, ro2, cm8) + dtx * d[m]);  // rk_tvd22
                   else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
                 }
               }
               o9[8] = glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
             }
 #pragma unroll
-            for (int m = 0; m < NV; m++) p.out[m * plane + o] = o9[m];
+            for (int m = 0; m < NV; m++) lane_at(p.out, plane, m, ro2, cm8) = o9[m];
             if constexpr (STAGE == 2) {  // next step's u2v and set_dt on U^{n+1}
               double w2[NV];
               cons_to_prim(o9, w2);
@@ -493,17 +531,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 // bits as the host-dt path.
 constexpr int GLM_TAB_HALF = 64, GLM_TAB = 2 * GLM_TAB_HALF + 1;
 
-// Device dt: for each of n members, dt and the GLM factor from its reduced
-// CFL speed (set_dt.f90:52, glm_ss2.f90:27, flux_resistive.cuf:8), then the
-// speed slot is cleared for the next step's reduction.  The GLM factor comes
-// from the member's table of host exp values (glm_tab, glm_x0: GLM_TAB
-// doubles and the bits of x0 per member); an argument outside the table,
-// which a CFL-limited dt cannot produce, falls back to the device's exp.
+// Device dt: for each of n members (or only member `only` when it is >= 0),
+// dt and the GLM factor from its reduced CFL speed (set_dt.f90:52,
+// glm_ss2.f90:27, flux_resistive.cuf:8), then the speed slot is cleared for
+// the next step's reduction.  The GLM factor comes from the member's table of
+// host exp values (glm_tab, glm_x0: GLM_TAB doubles and the bits of x0 per
+// member); an argument outside the table, which a CFL-limited dt cannot
+// produce, falls back to the device's exp.
 __global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const double *dx, const double *cfl,
                             const int *tt, const double *eta_max, const double *glm_tab, const long long *glm_x0,
-                            int *err, int n) {
+                            int *err, int n, int only = -1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (only >= 0 && i != only)) return;
   const double vmax = __longlong_as_double((long long)vmax_bits[i]);
   double dt = cfl[i] * dx[i] / vmax;
   if (!(dt >= 1.0e-7)) {
@@ -1194,9 +1233,10 @@ struct omhd_batch {
   std::vector<unsigned long long> vmax_h;
   std::vector<double> vmax;  // CFL speed of each member's current state
   std::vector<double> t;
+  std::vector<char> uploaded;  // members given a state by omhd_batch_upload
   hipStream_t stream = nullptr;
   int rows = 24, grid = 0;
-  bool device_dt = false, use_graph = false, have_state = false;
+  bool device_dt = false, use_graph = false, have_state = false;  // have_state: every member uploaded
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int lim = 1, tt = 0;
@@ -1265,6 +1305,7 @@ extern "C" int omhd_batch_create(const omhd_config *cfgs, int32_t n, omhd_batch 
   b->vmax_h.resize(n);
   b->vmax.assign(n, -1.0);
   b->t.assign(n, 0.0);
+  b->uploaded.assign(n, 0);
   std::vector<size_t> off(n);
   size_t total = 0;
   for (int i = 0; i < n; i++) {
@@ -1398,7 +1439,18 @@ extern "C" int omhd_batch_upload(omhd_batch *b, int32_t i, const double *U) {
   if (int rc = batch_check_err(b)) return rc;
   memcpy(&b->vmax[i], &v, sizeof(double));
   b->t[i] = 0.0;
-  b->have_state = true;
+  if (b->device_dt) {
+    // device-dt mode keeps the step parameters on the device: this member's
+    // next dt and GLM factor come from the speed just reduced (which also
+    // clears its slot for the stage-2 reduction), and its time restarts at 0
+    OMHD_HIP(hipMemsetAsync(b->tsum_d + i, 0, sizeof(double), b->stream));
+    k_device_dt<<<(b->n + 63) / 64, 64, 0, b->stream>>>(b->par_d, b->vmax_d, b->dx_d, b->cfl_d, b->tt_d, nullptr,
+                                                        b->glm_tab_d, b->glm_x0_d, b->err_d, b->n, i);
+    OMHD_HIP(hipGetLastError());
+    if (int rc = batch_check_err(b)) return rc;
+  }
+  b->uploaded[i] = 1;
+  b->have_state = std::all_of(b->uploaded.begin(), b->uploaded.end(), [](char c) { return c != 0; });
   return ok();
 }
 
@@ -1483,7 +1535,7 @@ static int batch_device_step(omhd_batch *b) {
 
 extern "C" int omhd_batch_step(omhd_batch *b, int32_t nsteps, double *dt_trace) {
   if (!b) return set_error(OMHD_ERR_ARG, "null batch");
-  if (!b->have_state) return set_error(OMHD_ERR_ARG, "upload the members first");
+  if (!b->have_state) return set_error(OMHD_ERR_ARG, "upload every member first");
   const double cr = 0.2;  // glm_ss2.f90:18
   if (b->device_dt) {
     if (b->use_graph && !b->gexec) {

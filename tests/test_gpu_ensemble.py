@@ -127,3 +127,37 @@ def test_benchmark_ensemble_members_equal_solo_runs():
             assert bits_equal(e.download(i), Us), (device_dt, i)
             assert bits_equal(e.times()[i], sum_in_order(dts)), (device_dt, i)
         e.close()
+
+
+def test_upload_in_device_dt_mode():
+    """A member (re)uploaded after the switch to device dt mode starts from its
+    new state with its own dt, GLM factor and time 0: after more steps it is
+    bitwise its solo run from that state, and the members left alone are
+    unaffected.  Stepping before every member has a state is refused."""
+    from openmhd_amd import Ensemble, OMHDError
+
+    cfgs, states = members(n=4, seed=9)
+    e = Ensemble(cfgs)
+    for i, U in enumerate(states[:-1]):
+        e.upload(i, U)
+    with pytest.raises(OMHDError):
+        e.step(1)  # the last member has no state yet
+    e.upload(len(states) - 1, states[-1])
+    e.set_mode(device_dt=True, graph=True)
+    e.step(4)
+    # member 1 restarts from member 1's state of a different ensemble draw
+    # (same grid), with the speeds, dt and GLM factor of that state
+    from openmhd_amd import problems
+
+    p = problems.kh_ensemble(4, seed=9)[1]
+    U_new, _ = problems.kh_member(dict(p, seed=p["seed"] + 1000))
+    e.upload(1, U_new)
+    e.step(5)
+    dts, Us = solo(cfgs[1], U_new, 5)
+    assert bits_equal(e.download(1), Us)
+    assert bits_equal(e.times()[1], sum_in_order(dts))
+    for i in (0, 2, 3):  # 9 steps since their upload
+        dts, Us = solo(cfgs[i], states[i], 9)
+        assert bits_equal(e.download(i), Us), i
+        assert bits_equal(e.times()[i], sum_in_order(dts)), i
+    e.close()
