@@ -51,3 +51,18 @@ def div_b_l2(U, dx, dy=None, dz=None) -> float:
                + (bz[1:-1, 1:-1, 2:] - bz[1:-1, 1:-1, :-2]) / (2 * dz))
         dv = dx * dy * dz
     return float(np.sqrt(np.sum(div * div) * dv))
+
+
+def reconnected_flux(W, dx, j0=None):
+    """Reconnected flux of a 3D current sheet centred at y = 0: half the
+    z-averaged unsigned By flux through the sheet plane,
+    phi = 1/2 * sum_{i,k} |By_c(i, k)| dx / nz, with By_c the mean of the two
+    rows that straddle y = 0 (rows j0-1 and j0, j0 = ny/2).  W is the
+    interior state (9, nz, ny, nx), x fastest.  Returns (phi, w), where w holds
+    dphi/dBy on those two rows (the derivative of |.| taken with the sign of
+    By_c; as a (nz, nx) array, equal on both rows)."""
+    nz, ny = W.shape[1], W.shape[2]
+    j0 = ny // 2 if j0 is None else j0
+    byc = 0.5 * (W[6, :, j0 - 1, :] + W[6, :, j0, :])
+    phi = 0.5 * float(np.abs(byc).sum()) * dx / nz
+    return phi, 0.25 * np.sign(byc) * dx / nz
