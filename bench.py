@@ -126,11 +126,12 @@ def cpu_baseline(n: int, sample_steps: int):
                       f"(gcc -O2 -fopenmp, {threads} threads)"}
 
 
-def bench_ensemble(a):
+def bench_ensemble(a, steps=None):
     """256 independent KH runs, sides log-uniform in [64, 1024],
     300 steps, each with its own dt.  Batched (one launch per RK stage over all
     members, autotuned tiles; host dt and device dt + hipGraph) against the same
-    members run one after another with the single-grid step."""
+    members run one after another with the single-grid step.  Returns the
+    figures as a dict."""
     import torch
 
     from openmhd_amd import Ensemble, Solver, make_config, problems
@@ -144,7 +145,7 @@ def bench_ensemble(a):
         cfgs.append(make_config(p["nx"], p["ny"], dx, cfl=0.4, lm_type=1, bc=("periodic", "wall")))
     cells = sum(p["nx"] * p["ny"] for p in params)
     log(f"ensemble: {len(params)} members, {cells} cells")
-    steps = a.steps
+    steps = a.steps if steps is None else steps
     seq = 0.0
     for U, c in zip(states, cfgs):
         s = Solver(c.nx, c.ny, c.dx, cfl=0.4, lm_type=1, bc=("periodic", "wall"))
@@ -174,23 +175,27 @@ def bench_ensemble(a):
         out[mode] = {"seconds": el, "zone_updates_per_s": cells * steps / el, "speedup_vs_sequential": seq / el,
                      "tile_rows": rows, "grid": grid}
     out["sequential_zone_updates_per_s"] = cells * steps / seq
-    print(json.dumps({"workload": "KH ensemble, 256 members, sides 64-1024, 300 steps", **out}), flush=True)
+    return {"workload": f"KH ensemble, {len(params)} members, sides 64-1024, {steps} steps", **out}
 
 
-def bench_3d(a):
+def bench_3d(a, steps=None, warmup=None, grad_steps=100, every=10, fd=None):
     """3D resistive Harris sheet 512x256x256 (Bx = tanh(y), rho =
     0.2 + cosh^-2(y)/0.2, eta = 1e-3 + 1e-2 exp(-(x^2+y^2)/4), 1e-3 velocity
     noise), HLLD + resistive flux + GLM + TVD-RK2, minmod, x/z periodic, y
     walls: forward zone-updates/s over `steps` steps; then (unless --no-grad)
-    the gradient of the reconnected flux at the end of 100 steps with respect
-    to (eta1, w) by the adjoint with checkpoints every 10 steps, checked by
-    central differences with the frozen dt sequence."""
+    the gradient of the reconnected flux at the end of `grad_steps` (100) steps
+    with respect to (eta1, w) by the adjoint with checkpoints every `every`
+    (10) steps, checked (unless --no-fd) by central differences with the
+    frozen dt sequence.  Returns the figures as a dict."""
     import numpy as np
     import torch
 
     from openmhd_amd import Adjoint, Solver, problems
 
     torch.cuda.set_device(0)
+    steps = a.steps if steps is None else steps
+    warmup = a.warmup if warmup is None else warmup
+    fd_on = (not a.no_fd) if fd is None else fd
     nx, ny, nz, dx = a.nx3, a.ny3, a.nz3, 0.125
     eta0, eta1, w = 1e-3, 1e-2, 2.0
     U, origin = problems.harris_sheet_3d(nx, ny, nz, dx)
@@ -199,28 +204,25 @@ def bench_3d(a):
     s.upload(U)
     W0 = s.get_state()
     del U
-    s.step(a.warmup)
+    s.step(warmup)
     s.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    s.step(a.steps)
+    s.step(steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms, n = s.timing()
     cells = nx * ny * nz
-    res = {"workload": f"3D {'ideal' if a.no_res else 'resistive'} Harris sheet {nx}x{ny}x{nz}", "steps": a.steps, "ms_per_step": 1e3 * el / a.steps,
-           "zone_updates_per_s": cells * a.steps / el,
-           "hbm_frac_step": cells * a.steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+    res = {"workload": f"3D {'ideal' if a.no_res else 'resistive'} Harris sheet {nx}x{ny}x{nz}", "steps": steps,
+           "ms_per_step": 1e3 * el / steps, "zone_updates_per_s": cells * steps / el,
+           "hbm_frac_step": cells * steps / el * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
            "stage_ms": [ms[0] / max(1, n[0]), ms[1] / max(1, n[1])]}
     log(f"forward: {res['ms_per_step']:.3f} ms/step")
     if not a.no_grad and not a.no_res:
-        nsteps, every = 100, 10
-        j0 = ny // 2  # rows j0-1 and j0 straddle the sheet centre y = 0
+        nsteps = grad_steps
+        from openmhd_amd.diagnostics import reconnected_flux
 
-        def flux(W):
-            """reconnected flux: half the z-averaged unsigned By flux through y = 0"""
-            byc = 0.5 * (W[6, :, j0 - 1, :] + W[6, :, j0, :])
-            return 0.5 * np.abs(byc).sum() * dx / nz, byc
+        j0 = ny // 2  # rows j0-1 and j0 straddle the sheet centre y = 0
 
         s.set_state(W0)
         adj = Adjoint(s, nsteps, every)
@@ -230,9 +232,8 @@ def bench_3d(a):
         torch.cuda.synchronize()
         t_fwd = time.perf_counter() - t0
         WT = s.get_state()
-        phi, byc = flux(WT)
+        phi, g_by = reconnected_flux(WT, dx)
         lamT = np.zeros_like(WT)
-        g_by = 0.5 * 0.5 * np.sign(byc) * dx / nz
         lamT[6, :, j0 - 1, :] = g_by
         lamT[6, :, j0, :] = g_by
         del WT
@@ -246,13 +247,11 @@ def bench_3d(a):
         log(f"adjoint: forward {t_fwd:.3f} s, backward {t_bwd:.3f} s, dphi/d(eta0, eta1, w) = {g}")
         # central differences of the flux with the sign pattern of the base run
         # (the same derivative at the base point; no |By| kink is crossed)
-        sgn_c = np.sign(byc)
-
         def flux_lin(W):
-            return 0.5 * (sgn_c * 0.5 * (W[6, :, j0 - 1, :] + W[6, :, j0, :])).sum() * dx / nz
+            return float((g_by * (W[6, :, j0 - 1, :] + W[6, :, j0, :])).sum())
 
         fd = {}
-        eps_list = () if a.no_fd else (1e-2, 1e-3, 1e-4, 1e-5)
+        eps_list = (1e-2, 1e-3, 1e-4, 1e-5) if fd_on else ()
         for k, name in ((1, "eta1"), (2, "w")):
             rel = []
             for eps in eps_list:
@@ -273,23 +272,25 @@ def bench_3d(a):
                     "dphi_deta1": g[1], "dphi_dw": g[2], "fd_eps": list(eps_list), "fd_rel_err": fd,
                     "adjoint_forward_s": t_fwd, "adjoint_backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd})
     s.close()
-    print(json.dumps(res), flush=True)
+    return res
 
 
-def bench_adjoint(a):
+def bench_adjoint(a, fd=True, nsteps=None):
     """Orszag-Tang 512^2 + 16 seeded Fourier velocity modes
     (|k| <= 8, amplitude 1e-2), 200 MC/HLLD/GLM/TVD-RK2 steps, checkpoints
     every 20; loss = ||By(T) - By_target(T)||^2 with the target run from a
     second seeded perturbation; gradient w.r.t. the 16 mode amplitudes via the
     adjoint (torch.autograd through Rollout), checked by central differences
-    along 16 seeded directions with the frozen dt/ch sequence."""
+    along 16 seeded directions with the frozen dt/ch sequence (unless fd is
+    False).  Returns the figures as a dict."""
     import numpy as np
     import torch
 
     from openmhd_amd import Adjoint, Rollout, Solver, problems
 
     torch.cuda.set_device(0)
-    n, nsteps, every = a.n if a.n != 4096 else 512, a.steps, 20
+    n, every = a.n if a.n != 4096 else 512, 20
+    nsteps = a.steps if nsteps is None else nsteps
     dx = 2 * np.pi / n
     s = Solver(n, n, dx, cfl=0.4, lm_type=2)
     modes = problems.fourier_modes(1)
@@ -317,7 +318,7 @@ def bench_adjoint(a):
     eps_list = (1e-7, 1e-8, 1e-9, 1e-10)
     rel = []
     with torch.no_grad():
-        for k in range(16):
+        for k in range(16 if fd else 0):
             d = rng.standard_normal(16)
             d /= np.linalg.norm(d)
             dv = torch.tensor(d, device="cuda")
@@ -329,8 +330,8 @@ def bench_adjoint(a):
                 r.append(abs((lp - lm) / (2 * eps) - an) / abs(an))
             rel.append(r)
             log(f"direction {k}: adjoint {an:.12e} rel. diff. at eps {eps_list}: " + " ".join(f"{x:.1e}" for x in r))
-    rel = np.array(rel)
-    best = rel.min(axis=1)
+    rel = np.array(rel).reshape(-1, len(eps_list))
+    best = rel.min(axis=1) if len(rel) else np.full(1, np.nan)
     # timing: plain steps, the adjoint's forward pass (with checkpoints), the backward pass
     W0 = problems.orszag_tang_perturbed(n, modes, np.full(16, 1e-2))
     W0d = torch.tensor(W0, device="cuda")
@@ -363,13 +364,39 @@ def bench_adjoint(a):
         t["backward"] = time.perf_counter() - t0
     adj.close()
     res = {"workload": f"adjoint: Orszag-Tang {n}^2 + 16 modes, {nsteps} steps, checkpoints every {every}",
-           "loss": float(L), "grad_norm": float(np.linalg.norm(grad)), "fd_eps": list(eps_list),
-           "fd_rel_err_median_per_eps": [float(x) for x in np.median(rel, axis=0)],
-           "fd_rel_err_best_max": float(best.max()), "fd_rel_err_best_median": float(np.median(best)),
-           "seconds": t, "backward_over_forward": t["backward"] / t["forward"],
+           "loss": float(L), "grad_norm": float(np.linalg.norm(grad)), "seconds": t,
+           "backward_over_forward": t["backward"] / t["forward"],
            "backward_over_plain_step": t["backward"] / t["plain_cfl"],
            "zone_updates_per_s_forward": n * n * nsteps / t["forward"]}
-    print(json.dumps(res), flush=True)
+    if fd:
+        res.update({"fd_eps": list(eps_list), "fd_rel_err_median_per_eps": [float(x) for x in np.median(rel, axis=0)],
+                    "fd_rel_err_best_max": float(best.max()), "fd_rel_err_best_median": float(np.median(best))})
+    return res
+
+
+def secondary(a):
+    """The other configurations, bounded so that the default run stays within
+    a few minutes: the 256-member ensemble (300 steps, both dt modes), the
+    Orszag-Tang 512^2 adjoint (200 steps, checkpoints every 20; the
+    finite-difference check is in --workload adjoint and the GPU tests), and
+    the 3D resistive Harris sheet at 512x256x256 (forward over 10 steps after
+    2, adjoint over 10 steps with checkpoints every 10)."""
+    import torch
+
+    out = {}
+    for name, fn in (("ensemble", lambda: bench_ensemble(a, steps=300)),
+                     ("adjoint_2d", lambda: bench_adjoint(a, fd=False, nsteps=200)),
+                     ("resistive_3d", lambda: bench_3d(a, steps=10, warmup=2, grad_steps=10, every=10, fd=False))):
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as e:  # a secondary figure never hides the headline
+            log(f"secondary {name} failed:", repr(e))
+            out[name] = {"error": repr(e)}
+        out[name]["wall_s"] = time.perf_counter() - t0
+        torch.cuda.empty_cache()
+        log(f"secondary {name}: {out[name]['wall_s']:.1f} s")
+    return out
 
 
 def main():
@@ -393,6 +420,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="3d: forward timing only")
     ap.add_argument("--no-fd", action="store_true", help="3d: skip the finite-difference check of the gradient")
     ap.add_argument("--no-res", action="store_true", help="3d: ideal MHD (no resistive flux), forward only")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="kh: skip the secondary configurations (ensemble, 2D adjoint, 3D resistive)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (rocprofv3 FETCH/WRITE_SIZE calibration)")
     a = ap.parse_args()
@@ -404,7 +433,9 @@ def main():
 
     if a.workload != "kh":
         sys.path.insert(0, str(REPO))
-        return {"ensemble": bench_ensemble, "3d": bench_3d, "adjoint": bench_adjoint}[a.workload](a)
+        res = {"ensemble": bench_ensemble, "3d": bench_3d, "adjoint": bench_adjoint}[a.workload](a)
+        print(json.dumps(res), flush=True)
+        return
 
     # the CPU baseline runs first, as a child process started before this
     # process initialises the GPU
@@ -482,6 +513,9 @@ def main():
         "cpu_baseline": cpu,
     }
     s.close()
+    del s
+    if not a.no_secondary:
+        out["secondary"] = secondary(a)
     print(json.dumps(out), flush=True)
 
 
