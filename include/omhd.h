@@ -94,6 +94,17 @@ int omhd_rk(int32_t mode, double *wk, double *wk1, const double *wF, const doubl
 int omhd_set_dt(const double *U, const double *V, double *vmax, double *dt, double dx, double cfl, int32_t ix,
                 int32_t jx, int32_t kx, int32_t ndim);
 
+/* omhd_set_dt without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as omhd_set_dt, except that U and V must be
+ * device memory (OMHD_ERR_ARG otherwise) and the grid may hold at most
+ * 67108864 interior cells (OMHD_ERR_EXTENT otherwise).  The reduction uses a
+ * fixed 512 KiB scratch array that the library keeps on each GPU for the
+ * life of the process; calls that overlap on one GPU share it and race. */
+int omhd_set_dt_noalloc(const double *U, const double *V, double *vmax, double *dt, double dx, double cfl,
+                        int32_t ix, int32_t jx, int32_t kx, int32_t ndim);
+
 /* bc(U,ix,jx[,kx]) / bc_for_U(U,ix,jx[,kx]) -- 2D_basic/bc.f90:1,
  * 2D_KH/bc.f90:6, 2D_reconnection/bc.f90:6, 3D_basic/bc.f90:1,
  * 3D_reconnection_gpu/bc.cuf:7.  bc_type[6] per face (x-,x+,y-,y+,z-,z+),

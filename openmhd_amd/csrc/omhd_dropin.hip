@@ -164,6 +164,87 @@ __global__ __launch_bounds__(256) void k_speed(const double *__restrict__ U, con
 }
 
 // ---------------------------------------------------------------------------
+// set_dt without allocation (omhd_set_dt_noalloc).  The scratch array holds
+// one partial per workgroup: pass 1 has each workgroup take
+// NOALLOC_CELLS_PER_GROUP interior cells and store its maximum speed and its
+// flags in its own slots; pass 2, one workgroup, reduces the slots of the
+// workgroups pass 1 ran.  Every slot a call reads is written earlier in the
+// same call, so nothing needs clearing, and no atomics are used.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_GROUPS = 32768;
+constexpr int NOALLOC_CELLS_PER_THREAD = 8;
+constexpr size_t NOALLOC_CELLS_PER_GROUP = (size_t)TPB * NOALLOC_CELLS_PER_THREAD;
+constexpr size_t NOALLOC_MAX_CELLS = (size_t)NOALLOC_GROUPS * NOALLOC_CELLS_PER_GROUP;  // 67,108,864
+// [0, G): max-speed bits per workgroup; [G, 2G): flags per workgroup;
+// [2G], [2G+1]: the reduced max-speed bits and flags.  512 KiB + 16 B per GPU.
+__device__ __align__(256) unsigned long long scratch[2 * NOALLOC_GROUPS + 2];
+
+// workgroup-wide max of non-negative doubles and OR of flags (TPB threads)
+__device__ __forceinline__ void group_reduce(double &v, int &f) {
+  __shared__ double wv[TPB / 64];
+  __shared__ int wf[TPB / 64];
+  v = wave_max(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, 64);
+  if ((threadIdx.x & 63) == 0) {
+    wv[threadIdx.x >> 6] = v;
+    wf[threadIdx.x >> 6] = f;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < TPB / 64; w++) {
+      v = dmax(v, wv[w]);
+      f |= wf[w];
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_speed_part(const double *__restrict__ U, const double *__restrict__ V, int ix,
+                                                    int jx, int kx, unsigned long long *part) {
+  const int is = ix > 1 ? 1 : 0, js = jx > 1 ? 1 : 0, ks = kx > 1 ? 1 : 0;
+  const int ni = ix > 2 ? ix - 2 : 1, nj = jx > 2 ? jx - 2 : 1, nk = kx > 2 ? kx - 2 : 1;
+  const size_t total = (size_t)ni * nj * nk, n = (size_t)ix * jx * kx;
+  double vt = 0.0;
+  int f = 0;
+  for (int q = 0; q < NOALLOC_CELLS_PER_THREAD; q++) {
+    const size_t gid = (size_t)blockIdx.x * NOALLOC_CELLS_PER_GROUP + (size_t)q * TPB + threadIdx.x;
+    if (gid >= total) break;
+    const int i = is + (int)(gid % ni);
+    const size_t r = gid / ni;
+    const int j = js + (int)(r % nj), k = ks + (int)(r / nj);
+    const size_t c = (size_t)i + (size_t)ix * ((size_t)j + (size_t)jx * k);
+    double u[NV], w[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) u[m] = U[m * n + c];
+#pragma unroll
+    for (int m = 0; m < 4; m++) w[m] = V[m * n + c];
+    const double c1 = fastest_speed<ND>(u, w);
+    if (!(c1 >= 0.0)) f |= 4;  // NaN
+    else vt = dmax(vt, c1);
+  }
+  group_reduce(vt, f);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = (unsigned long long)__double_as_longlong(vt);
+    part[NOALLOC_GROUPS + blockIdx.x] = (unsigned long long)f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_speed_final(const unsigned long long *__restrict__ part, int groups,
+                                                     unsigned long long *out) {
+  double vt = 0.0;
+  int f = 0;
+  for (int g = threadIdx.x; g < groups; g += TPB) {
+    vt = dmax(vt, __longlong_as_double((long long)part[g]));
+    f |= (int)part[NOALLOC_GROUPS + g];
+  }
+  group_reduce(vt, f);
+  if (threadIdx.x == 0) {
+    out[0] = (unsigned long long)__double_as_longlong(vt);
+    out[1] = (unsigned long long)f;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ghost fill, one face pair per launch (x, then y, then z: bc.f90 order)
 // ---------------------------------------------------------------------------
 __global__ void k_bc_face_pair(double *__restrict__ U, int ix, int jx, int kx, int dir, int type_lo, int type_hi,
@@ -549,6 +630,57 @@ extern "C" int omhd_set_dt(const double *U, const double *V, double *vmax, doubl
   (void)hipFreeAsync(red, s);
   a.release(s);
   b.release(s);
+  double vm;
+  memcpy(&vm, &h[0], sizeof vm);
+  *vmax = vm;
+  *dt = cfl * dx / vm;  // set_dt.f90:52
+  if ((int)h[1] & 4) return set_error(OMHD_ERR_DT, "non-finite signal speed", 0, 0, 0, vm);
+  if (!(*dt >= 1.0e-7)) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, *dt);  // :55-65
+  return ok();
+}
+
+// Device address of this GPU's `scratch`, looked up once per device (the
+// array's name in host code is only a stand-in for the symbol).
+static int noalloc_scratch(unsigned long long **out) {
+  static std::mutex mu;
+  static unsigned long long *addr[64] = {};
+  int dev = 0;
+  OMHD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_error(OMHD_ERR_ARG, "device index out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    OMHD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(OMHD_ERR_HIP, "no device address for the set_dt scratch array");
+    addr[dev] = (unsigned long long *)p;
+  }
+  *out = addr[dev];
+  return OMHD_OK;
+}
+
+extern "C" int omhd_set_dt_noalloc(const double *U, const double *V, double *vmax, double *dt, double dx, double cfl,
+                                   int32_t ix, int32_t jx, int32_t kx, int32_t ndim) {
+  if (!U || !V || !vmax || !dt) return set_error(OMHD_ERR_ARG, "null argument");
+  if (int rc = check_ext(ix, jx, kx)) return rc;
+  if (!is_device_ptr(U) || !is_device_ptr(V))
+    return set_error(OMHD_ERR_ARG, "omhd_set_dt_noalloc: U and V must be device memory (host arrays need staging)");
+  const int ni = ix > 2 ? ix - 2 : 1, nj = jx > 2 ? jx - 2 : 1, nk = kx > 2 ? kx - 2 : 1;
+  const size_t total = (size_t)ni * nj * nk;
+  if (total > NOALLOC_MAX_CELLS)
+    return set_error(OMHD_ERR_EXTENT, "omhd_set_dt_noalloc: more than 67108864 interior cells (use omhd_set_dt)", ix, jx,
+                     kx);
+  unsigned long long *sc = nullptr;
+  if (int rc = noalloc_scratch(&sc)) return rc;
+  hipStream_t s = dropin_stream();
+  const unsigned groups = (unsigned)((total + NOALLOC_CELLS_PER_GROUP - 1) / NOALLOC_CELLS_PER_GROUP);
+  if (ndim == 3) k_speed_part<3><<<groups, TPB, 0, s>>>(U, V, ix, jx, kx, sc);
+  else k_speed_part<2><<<groups, TPB, 0, s>>>(U, V, ix, jx, kx, sc);
+  OMHD_HIP(hipGetLastError());
+  k_speed_final<<<1, TPB, 0, s>>>(sc, (int)groups, sc + 2 * NOALLOC_GROUPS);
+  OMHD_HIP(hipGetLastError());
+  unsigned long long h[2];
+  OMHD_HIP(hipMemcpyAsync(h, sc + 2 * NOALLOC_GROUPS, sizeof h, hipMemcpyDeviceToHost, s));
+  OMHD_HIP(hipStreamSynchronize(s));
   double vm;
   memcpy(&vm, &h[0], sizeof vm);
   *vmax = vm;
