@@ -256,6 +256,22 @@ int omhd_adjoint_backward(omhd_adjoint *a, const double *lam_T, double *lam_0);
 int omhd_adjoint_eta_grad(omhd_adjoint *a, double *grad3);
 int omhd_adjoint_destroy(omhd_adjoint *a);
 
+/* Device mirrors of host arrays for the drop-in entry points: with mirroring
+ * on, a host array passed to any entry point above keeps a device copy from
+ * call to call (keyed by its host address range), so an unchanged driver that
+ * touches its arrays only through these routines and the snapshot routines
+ * runs with its state resident in HBM.  Host copies then go stale: only
+ * omhd_mirror_sync brings them up to date, and a host write must be followed
+ * by omhd_mirror_update.  The shims of fileio_output / fileio_input
+ * (2D_basic/fileio.f90:16,58; openmhd_amd/fortran/omhd_fileio.f90) do both.
+ * Off by default; OMHD_MIRROR=1 in the environment turns it on for the whole
+ * process (drivers such as 2D_basic/main.f90, whose lines 101-102 patch VL/VR
+ * on the host between calls, must run without it).  omhd_mirror_mode(0) writes
+ * every mirror back to the host and frees them. */
+int omhd_mirror_mode(int32_t on);
+int omhd_mirror_sync(const double *host, int64_t count);
+int omhd_mirror_update(const double *host, int64_t count);
+
 /* Library version and the gfx target it was built for. */
 const char *omhd_version(void);
 

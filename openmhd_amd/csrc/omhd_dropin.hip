@@ -5,7 +5,12 @@
 // get the reference's results; the fused kernels in omhd_step.hip are the
 // fast path.  One lane per cell (or interface).
 #include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "omhd_device.h"
 #include "omhd_host.h"
@@ -15,6 +20,106 @@ namespace omhd {
 omhd_error_info &tls_error() {
   static thread_local omhd_error_info e{};
   return e;
+}
+
+// ---------------------------------------------------------------------------
+// Device mirrors of host arrays.  A mirror holds one contiguous host address
+// range [lo, hi) in device memory.  A drop-in call on a host array inside a
+// mirror works on the mirror; one that reaches outside every mirror gets a
+// new mirror covering its range and every mirror it overlaps, uploaded from
+// the host where no mirror held the bytes and copied from the old mirrors
+// where one did.  dev_newer: a call has written the mirror since the host
+// copy was last brought up to date.  All copies are synchronous: every
+// drop-in entry point drains its stream before returning, so no kernel is
+// pending on a mirror when the next call looks at it.
+// ---------------------------------------------------------------------------
+namespace {
+struct Mirror {
+  uintptr_t lo, hi;
+  char *dev;
+  bool dev_newer;
+};
+std::mutex mirror_mu;
+std::map<uintptr_t, Mirror> mirrors;  // by lo
+int mirror_state = -1;                // -1: not decided yet (OMHD_MIRROR in the environment decides)
+
+// mirrors overlapping [lo, hi), in address order
+std::vector<std::map<uintptr_t, Mirror>::iterator> overlapping(uintptr_t lo, uintptr_t hi) {
+  std::vector<std::map<uintptr_t, Mirror>::iterator> out;
+  auto it = mirrors.upper_bound(lo);
+  if (it != mirrors.begin()) {
+    auto prev = std::prev(it);
+    if (prev->second.hi > lo) out.push_back(prev);
+  }
+  for (; it != mirrors.end() && it->second.lo < hi; ++it) out.push_back(it);
+  return out;
+}
+
+int mirror_release_all(bool write_back) {
+  for (auto &kv : mirrors) {
+    Mirror &m = kv.second;
+    if (write_back && m.dev_newer)
+      OMHD_HIP(hipMemcpy((void *)m.lo, m.dev, m.hi - m.lo, hipMemcpyDeviceToHost));
+    OMHD_HIP(hipFree(m.dev));
+  }
+  mirrors.clear();
+  return OMHD_OK;
+}
+}  // namespace
+
+bool mirror_on() {
+  std::lock_guard<std::mutex> lock(mirror_mu);
+  if (mirror_state < 0) {
+    const char *e = getenv("OMHD_MIRROR");
+    mirror_state = (e && atoi(e) != 0) ? 1 : 0;
+  }
+  return mirror_state > 0;
+}
+
+int mirror_acquire(const double *p, size_t count, bool writes, double **dev) {
+  std::lock_guard<std::mutex> lock(mirror_mu);
+  const uintptr_t lo = (uintptr_t)p, hi = lo + count * sizeof(double);
+  auto ov = overlapping(lo, hi);
+  if (ov.size() == 1 && ov[0]->second.lo <= lo && hi <= ov[0]->second.hi) {
+    Mirror &m = ov[0]->second;
+    *dev = (double *)(m.dev + (lo - m.lo));
+    m.dev_newer = m.dev_newer || writes;
+    return OMHD_OK;
+  }
+  uintptr_t nlo = lo, nhi = hi;
+  bool newer = writes;
+  for (auto &it : ov) {
+    nlo = std::min(nlo, it->second.lo);
+    nhi = std::max(nhi, it->second.hi);
+  }
+  char *d = nullptr;
+  OMHD_HIP(hipMalloc((void **)&d, nhi - nlo));
+  // host bytes first, then every old mirror's bytes over them
+  OMHD_HIP(hipMemcpy(d, (const void *)nlo, nhi - nlo, hipMemcpyHostToDevice));
+  for (auto &it : ov) {
+    Mirror &m = it->second;
+    OMHD_HIP(hipMemcpy(d + (m.lo - nlo), m.dev, m.hi - m.lo, hipMemcpyDeviceToDevice));
+    newer = newer || m.dev_newer;
+    OMHD_HIP(hipFree(m.dev));
+    mirrors.erase(it);
+  }
+  mirrors[nlo] = Mirror{nlo, nhi, d, newer};
+  *dev = (double *)(d + (lo - nlo));
+  return OMHD_OK;
+}
+
+int mirror_sync(const double *p, size_t count) {
+  std::lock_guard<std::mutex> lock(mirror_mu);
+  const uintptr_t lo = (uintptr_t)p, hi = lo + count * sizeof(double);
+  for (auto &it : overlapping(lo, hi)) {
+    Mirror &m = it->second;
+    if (!m.dev_newer) continue;
+    // the whole mirror: host bytes outside [lo, hi) but inside it are the
+    // same arrays, and the mirror holds their newest values too
+    OMHD_HIP(hipMemcpy((void *)m.lo, m.dev, m.hi - m.lo, hipMemcpyDeviceToHost));
+    m.dev_newer = false;
+  }
+  return OMHD_OK;
 }
 
 hipStream_t dropin_stream() {
@@ -397,6 +502,10 @@ void unflatten(size_t c, int ix, int jx, int &i, int &j, int &k) {
 
 // minloc of plane[0..n) held by the caller (host or device)
 size_t plane_minloc(const double *plane, size_t n, bool dev, double &val) {
+  if (!dev && mirror_on() && mirror_sync(plane, n) != OMHD_OK) {
+    val = 0.0;
+    return 0;  // the location is diagnostic only; the caller still reports the error code
+  }
   const double *h = plane;
   double *tmp = nullptr;
   if (dev) {
@@ -425,6 +534,37 @@ extern "C" int omhd_last_error(omhd_error_info *out) {
   if (!out) return OMHD_ERR_ARG;
   *out = tls_error();
   return OMHD_OK;
+}
+
+extern "C" int omhd_mirror_mode(int32_t on) {
+  {
+    std::lock_guard<std::mutex> lock(mirror_mu);
+    if (!on) {
+      if (int rc = mirror_release_all(true)) return rc;
+    }
+    mirror_state = on ? 1 : 0;
+  }
+  return ok();
+}
+
+extern "C" int omhd_mirror_sync(const double *host, int64_t count) {
+  if (!host || count < 0) return set_error(OMHD_ERR_ARG, "null array or negative count");
+  if (!mirror_on()) return ok();
+  if (int rc = mirror_sync(host, (size_t)count)) return rc;
+  return ok();
+}
+
+extern "C" int omhd_mirror_update(const double *host, int64_t count) {
+  if (!host || count < 0) return set_error(OMHD_ERR_ARG, "null array or negative count");
+  if (!mirror_on()) return ok();
+  std::lock_guard<std::mutex> lock(mirror_mu);
+  const uintptr_t lo = (uintptr_t)host, hi = lo + (size_t)count * sizeof(double);
+  for (auto &it : overlapping(lo, hi)) {  // upload the part of each mirror that the host rewrote
+    Mirror &m = it->second;
+    const uintptr_t a = std::max(lo, m.lo), b = std::min(hi, m.hi);
+    OMHD_HIP(hipMemcpy(m.dev + (a - m.lo), (const void *)a, b - a, hipMemcpyHostToDevice));
+  }
+  return ok();
 }
 
 extern "C" const char *omhd_version(void) { return "omhd 0.1 (gfx950, fp64, fused column-march sweeps)"; }

@@ -49,10 +49,23 @@ inline bool is_device_ptr(const void *p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-// A device view of a caller array: used in place when it is device memory,
-// otherwise a device copy (uploaded if `in`, downloaded at finish() if `out`).
-// A copy is freed, stream-ordered, by release() or when the view goes out of
-// scope, so an entry point that returns early on an error frees it too.
+// Device mirrors of host arrays (omhd_mirror_mode).  When mirroring is on, a
+// host array handed to a drop-in entry point keeps its device copy from call
+// to call instead of crossing the bus each time, and its host copy is brought
+// up to date only by omhd_mirror_sync (which the fileio shims call).
+bool mirror_on();
+// device address of host [p, p + count) in its mirror, made or grown as
+// needed (host contents are uploaded where no mirror holds the range);
+// `writes`: the call writes into the range, so the host copy goes stale
+int mirror_acquire(const double *p, size_t count, bool writes, double **dev);
+// bring host [p, p + count) up to date from the mirrors that hold it
+int mirror_sync(const double *p, size_t count);
+
+// A device view of a caller array: used in place when it is device memory;
+// for host memory, the array's mirror when mirroring is on, otherwise a
+// device copy (uploaded if `in`, downloaded at finish() if `out`).  A copy is
+// freed, stream-ordered, by release() or when the view goes out of scope, so
+// an entry point that returns early on an error frees it too.
 struct Staged {
   double *dev = nullptr;
   double *host = nullptr;
@@ -72,6 +85,7 @@ struct Staged {
       return OMHD_OK;
     }
     host = const_cast<double *>(p);
+    if (mirror_on()) return mirror_acquire(p, count, out_, &dev);  // not owned: the mirror outlives the call
     OMHD_HIP(hipMallocAsync((void **)&dev, n * sizeof(double), s));
     owned = true;
     if (in) OMHD_HIP(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, s));

@@ -131,6 +131,20 @@ module omhd_iface
        import :: c_int, c_ptr
        type(c_ptr), value :: h
      end function
+     integer(c_int) function omhd_mirror_mode(on) bind(C, name='omhd_mirror_mode')
+       import :: c_int, c_int32_t
+       integer(c_int32_t), value :: on
+     end function
+     integer(c_int) function omhd_mirror_sync(a, n) bind(C, name='omhd_mirror_sync')
+       import :: c_int, c_int64_t, c_double
+       real(c_double), intent(in) :: a(*)
+       integer(c_int64_t), value :: n
+     end function
+     integer(c_int) function omhd_mirror_update(a, n) bind(C, name='omhd_mirror_update')
+       import :: c_int, c_int64_t, c_double
+       real(c_double), intent(in) :: a(*)
+       integer(c_int64_t), value :: n
+     end function
   end interface
 
 contains

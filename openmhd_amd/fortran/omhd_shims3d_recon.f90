@@ -17,6 +17,7 @@
 ! caller with another profile edits eta and the two resistivity constants.
 !-----------------------------------------------------------------------
 module omhd_recon3d_resistivity
+  use omhd_iface, only: OMHD_OK, c_int64_t, omhd_mirror_update
   implicit none
   ! magnetic Reynolds numbers: localized (the peak of eta, 1/Rm1) and
   ! background (1/Rm0); flux_resistive.cuf:6, :41-42
@@ -46,6 +47,8 @@ contains
           eta_face(i, j, 3) = eta(x(i), y(j))
        enddo
     enddo
+    ! a new table may sit where an old one did: refresh a device mirror of it
+    if (omhd_mirror_update(eta_face, int(size(eta_face), c_int64_t)) /= OMHD_OK) stop 'omhd_mirror_update'
   end subroutine eta_tables3
 end module omhd_recon3d_resistivity
 

@@ -18,6 +18,7 @@
 ! caller with another profile edits eta and the two resistivity constants.
 !-----------------------------------------------------------------------
 module omhd_recon_resistivity
+  use omhd_iface, only: OMHD_OK, c_int64_t, omhd_mirror_update
   implicit none
   ! magnetic Reynolds numbers: localized (the peak of eta, 1/Rm1) and
   ! background (1/Rm0); flux_resistive.f90:5-7, :22
@@ -47,6 +48,8 @@ contains
           eta_dx(i, j, 2) = eta(x(i), y(j) + 0.5d0 * dx) * odx   ! :97
        enddo
     enddo
+    ! a new table may sit where an old one did: refresh a device mirror of it
+    if (omhd_mirror_update(eta_dx, int(size(eta_dx), c_int64_t)) /= OMHD_OK) stop 'omhd_mirror_update'
   end subroutine eta_tables
 end module omhd_recon_resistivity
 
