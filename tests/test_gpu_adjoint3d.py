@@ -164,9 +164,11 @@ def test_eta_parameter_gradient(ac, oracle):
 def test_rollout_takes_eta_as_a_tensor(ac, oracle):
     """Rollout(W0, eta=tensor): the resistivity parameters are a torch input.
     At parameters other than the solver's own, torch.autograd gives eta.grad
-    equal bit for bit to Adjoint.eta_grad() (and W0.grad to Adjoint.backward)
-    of the same run, and equal to the exact tangent along d eta/d eta1 and
-    d eta/d w to 1e-10."""
+    equal to Adjoint.eta_grad() (and W0.grad to Adjoint.backward) of the same
+    run to 1e-12 -- not bit for bit: the resistive reverse accumulates the B
+    and eta adjoints with f64 atomic adds, whose order varies from pass to
+    pass -- and equal to the exact tangent along d eta/d eta1 and d eta/d w
+    to 1e-10."""
     import torch
 
     from openmhd_amd import Adjoint, Rollout, snapshot
@@ -192,8 +194,9 @@ def test_rollout_takes_eta_as_a_tensor(ac, oracle):
     WT = roll(W0t, eta=etat)
     assert np.array_equal(roll.dt.view(np.int64), dts.view(np.int64))
     (WT * torch.tensor(lamT, device="cuda")).sum().backward()
-    assert np.array_equal(etat.grad.numpy().view(np.int64), g.view(np.int64)), (etat.grad, g)
-    assert np.array_equal(W0t.grad.cpu().numpy().view(np.int64), gW.view(np.int64))
+    assert np.abs(etat.grad.numpy() - g).max() <= 1e-12 * np.abs(g).max(), (etat.grad, g)
+    gWt = W0t.grad.cpu().numpy()
+    assert np.abs(gWt - gW).max() <= 1e-12 * np.abs(gW).max()
     # exact tangents at prm
     x = snapshot.grid(nx, 0.1, origin[0])
     y = snapshot.grid(ny, 0.1, origin[1])

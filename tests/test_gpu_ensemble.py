@@ -149,7 +149,7 @@ def test_upload_in_device_dt_mode():
     # (same grid), with the speeds, dt and GLM factor of that state
     from openmhd_amd import problems
 
-    p = problems.kh_ensemble(4, seed=9)[1]
+    p = problems.kh_ensemble(4, seed=9, smin=24, smax=140)[1]  # the draw members() made
     U_new, _ = problems.kh_member(dict(p, seed=p["seed"] + 1000))
     e.upload(1, U_new)
     e.step(5)
