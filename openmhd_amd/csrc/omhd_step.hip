@@ -117,11 +117,16 @@ struct SweepLds {
   double ring[3][NV][66];  // [slot][var][lane + 1]; entries 0 and 65 repeat lanes 0 and 63
   double wly[NV][64];
 };
-struct SweepLdsRes {
+struct SweepLdsRes {  // XY pass
   double ring[3][NV][66];
   double wly[NV][64];
-  double aux[3][4][66];  // XY: Bx(k-1), Bx(k+1), By(k-1), By(k+1); Z: Bz(j-1), Bz(j+1)
-  double bxm[66];        // XY: Bx of row rn-3
+  double aux[3][4][66];  // Bx(k-1), Bx(k+1), By(k-1), By(k+1)
+  double bxm[66];        // Bx of row rn-3
+};
+struct SweepLdsResZ {  // Z pass: 22.1 KB, 7 wavefronts per CU where the XY layout's 25.7 KB allows 6
+  double ring[3][NV][66];
+  double wly[NV][64];
+  double aux[3][2][66];  // Bz(j-1), Bz(j+1)
 };
 
 // Element of variable m at this lane's column of a row: `ro` is the row's
@@ -142,8 +147,9 @@ __device__ __forceinline__ void put66(double *row, int lane, double v) {
   if (lane == 63) row[65] = v;
 }
 
-template <int RES>
-using LdsOf = typename std::conditional<RES != 0, SweepLdsRes, SweepLds>::type;
+template <int RES, int PASS>
+using LdsOf = typename std::conditional<
+    RES != 0, typename std::conditional<PASS == PASS_Z, SweepLdsResZ, SweepLdsRes>::type, SweepLds>::type;
 
 // One tile: strip x chunk (x slice) of one grid.
 // PREF: stage 2 issues the U^n loads of the combined row early (where the
@@ -151,7 +157,7 @@ using LdsOf = typename std::conditional<RES != 0, SweepLdsRes, SweepLds>::type;
 // ensemble kernel, whose per-member parameters take more registers).
 // GEN: the flux is p.ftype's (LLF, HLL, HLLC, LHLLD) instead of HLLD.
 template <int LIM, int STAGE, int TT, int PASS, int RES, bool PREF = true, bool GEN = false>
-__device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RES> &lds) {
+__device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RES, PASS> &lds) {
   constexpr bool XS = PASS != PASS_Z;         // x-sweep of the oldest row
   constexpr bool FINAL = PASS != PASS_XY;     // RK update (else write acc)
   constexpr int MD = PASS == PASS_Z ? 2 : 1;  // march axis = direction of its HLLD
@@ -487,7 +493,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
 
 template <int LIM, int STAGE, int TT, int PASS, int RES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march(const Sweep p) {
-  __shared__ LdsOf<RES> lds;
+  __shared__ LdsOf<RES, PASS> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
   if (STAGE == 2 && p.err[0] != 0) return;  // stage 1 failed: leave U^n as it is
@@ -498,7 +504,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 // types (p.ftype), same structure
 template <int LIM, int STAGE, int TT, int PASS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march_gen(const Sweep p) {
-  __shared__ LdsOf<0> lds;
+  __shared__ LdsOf<0, PASS_2D> lds;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   if (t >= p.ntiles) return;
   if (STAGE == 2 && p.err[0] != 0) return;  // stage 1 failed: leave U^n as it is
@@ -510,7 +516,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 template <int LIM, int STAGE, int TT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_march_ens(
     const Sweep *__restrict__ members, const int2 *__restrict__ tiles, int ntiles) {
-  __shared__ LdsOf<0> lds;
+  __shared__ LdsOf<0, PASS_2D> lds;
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const int2 tl = tiles[w];
     Sweep p = members[tl.x];
