@@ -12,7 +12,8 @@
 //   lam*  = c1 lam** + (dD/dW at U*)^T (a2 lam**)
 //   lamg  = c0 lam** + lam* + (dD/dW at Ug)^T (a1 lam*)
 //   lam   = G lamg
-// Each (dD/dW)^T mu is 2 + (number of axes) launches over the grid:
+// Each (dD/dW)^T mu is 2 + (number of axes) launches over the grid (plus the
+// low wall faces, and the resistive kernels below):
 //   k_adj_prim     primitives of W (u2v)
 //   k_adj_face<x>  per x-face: recompute the two limited face states and the
 //                  HLLD flux, run its reverse on bF = mu(c+1) - mu(c), store the
@@ -25,8 +26,11 @@
 // Walls: a wall face's states are mirrors of the inside face state, so its
 // reverse folds both state adjoints onto the inside one; the slopes of the
 // two edge cells read the cell itself through its mirror ghost.
-// Resistive flux: its reverse adds the B adjoints of each face's ten-cell
-// stencil with atomics (stencils overlap) and the eta adjoint per face.
+// Resistive flux, owner-gather (no atomics, so the result does not depend on
+// the order threads run in): k_adj_res stores each face's current-density and
+// B-sum adjoints and its eta adjoint; k_adj_res_gather sums, per cell, the
+// contributions of the ten-cell stencils that read it; k_adj_eta sums each
+// face column's eta adjoints over k into the face's eta entry.
 // Branches (limiter min/sign, HLLD regions) are re-decided from the recomputed
 // values, i.e. the adjoint follows the branch the forward pass took.
 //
@@ -133,7 +137,7 @@ __device__ __forceinline__ double ld(const double *a, const Geo &g, const Cell &
 }
 
 __global__ __launch_bounds__(256) void k_adj_prim(const Geo g, const double *__restrict__ U, double psi_f,
-                                                  double *__restrict__ w, double *__restrict__ R) {
+                                                  double *__restrict__ w) {
   int i, j, k;
   if (!cell_of(g, i, j, k)) return;
   const long long o = (long long)k * g.zstride + (long long)j * g.pitch + i;
@@ -144,30 +148,43 @@ __global__ __launch_bounds__(256) void k_adj_prim(const Geo g, const double *__r
   adj::prim_fwd<double>(u, p);
 #pragma unroll
   for (int m = 0; m < NV; m++) w[m * g.plane + o] = p[m];
-  if (R) {  // resistive B adjoints, accumulated by the face kernels
-#pragma unroll
-    for (int m = 0; m < 3; m++) R[m * g.plane + o] = 0.0;
-  }
+}
+
+// Resistive reverse, owner-gather form (no atomics, fixed summation order).
+// Face arrays: per axis DIR, 5 planes over (nx + [DIR==0]) (ny + [DIR==1])
+// (nz + [DIR==2]) faces, the DIR coordinate of the face's left cell stored
+// shifted by one (left cell -1 .. n-1: the low wall face is -1).
+__host__ __device__ __forceinline__ long long fplane_of(const Geo &g, int d) {
+  return (long long)(g.nx + (d == 0)) * (g.ny + (d == 1)) * (g.nz + (d == 2));
+}
+__host__ __device__ __forceinline__ long long fbase_of(const Geo &g, int d) {
+  long long b = 0;
+  for (int e = 0; e < d; e++) b += 5 * fplane_of(g, e);
+  return b;
+}
+template <int DIR>
+__device__ __forceinline__ long long face_idx(const Geo &g, int i, int j, int k) {
+  return ((long long)(k + (DIR == 2)) * (g.ny + (DIR == 1)) + (j + (DIR == 1))) * (g.nx + (DIR == 0)) + (i + (DIR == 0));
 }
 
 // Reverse of the resistive flux at the face with left cell (i,j,k) along DIR
-// (3D_reconnection_gpu/flux_resistive.cuf:86-135): the B adjoints go to the
-// ten stencil cells (atomically: stencils of neighbouring faces overlap), the
-// eta adjoint to the face's entry of geta.  J1 pairs with slot q1 in the
-// energy flux and feeds slot q1's flux with +eta J1; J2 with q2 and -eta J2.
+// (3D_reconnection_gpu/flux_resistive.cuf:86-135), first half: the face's
+// current-density and B-sum adjoints (bJ1, bS1, bJ2, bS2) and its eta adjoint,
+// stored at the face.  J1 pairs with slot q1 in the energy flux and feeds slot
+// q1's flux with +eta J1; J2 with q2 and -eta J2.  The B adjoints of the
+// ten stencil cells are gathered by res_gather.
 template <int DIR>
-__device__ void resistive_vjp(const Geo &g, const double *__restrict__ w, int i, int j, int k, const double bF[NV],
-                              double *__restrict__ R, double *__restrict__ geta) {
+__device__ void resistive_face(const Geo &g, const double *__restrict__ w, int i, int j, int k, const double bF[NV],
+                               double *__restrict__ F) {
   constexpr int q1 = DIR == 0 ? 7 : (DIR == 1 ? 5 : 6);  // x: bz, y: bx, z: by
   constexpr int q2 = DIR == 0 ? 6 : (DIR == 1 ? 7 : 5);  // x: by, y: bz, z: bx
-  constexpr int t1 = DIR == 0 ? 5 : (DIR == 1 ? 6 : 7);  // slot of J1's transverse differences
+  constexpr int t1 = 5 + DIR;                             // slot of J1's and J2's transverse differences
   constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
   // J1's 0.25-terms vary along axis A1, J2's along A2 (x: z and y; y: x and z; z: y and x)
   constexpr int A1 = DIR == 0 ? 2 : (DIR == 1 ? 0 : 1);
   constexpr int A2 = DIR == 0 ? 1 : (DIR == 1 ? 2 : 0);
-  const int ix = g.nx + 2, jx = g.ny + 2;
-  const long long ei = (long long)DIR * ix * jx + (long long)(i + 1) + (long long)ix * (j + 1);
-  const double e = g.eta[ei] / g.dx;
+  const int ix = g.nx + 2;
+  const double e = g.eta[(long long)DIR * ix * (g.ny + 2) + (long long)(i + 1) + (long long)ix * (j + 1)] / g.dx;
   const Cell c0 = cell_at(g, i, j, k), c1 = cell_at(g, i + di, j + dj, k + dk);
   auto sh = [&](int a, int s, int base_i, int base_j, int base_k) {
     return cell_at(g, base_i + (a == 0 ? s : 0), base_j + (a == 1 ? s : 0), base_k + (a == 2 ? s : 0));
@@ -177,32 +194,107 @@ __device__ void resistive_vjp(const Geo &g, const double *__restrict__ w, int i,
   const Cell p2a = sh(A2, 1, i, j, k), p2b = sh(A2, 1, i + di, j + dj, k + dk);
   const Cell m2a = sh(A2, -1, i, j, k), m2b = sh(A2, -1, i + di, j + dj, k + dk);
   // J1 = -(q1(c+1) - q1(c)) + 0.25 (t1(+A1) terms - t1(-A1) terms)  [x: JyS, y: JzS, z: JxS]
-  // J2 =  (q2(c+1) - q2(c)) - 0.25 (t2(+A2) terms - t2(-A2) terms)  with t2 = t1's slot
+  // J2 =  (q2(c+1) - q2(c)) - 0.25 (t1(+A2) terms - t1(-A2) terms)
   const double J1 = -(ld(w, g, c1, q1) - ld(w, g, c0, q1)) +
                     0.25 * (ld(w, g, p1a, t1) + ld(w, g, p1b, t1) - ld(w, g, m1a, t1) - ld(w, g, m1b, t1));
   const double J2 = (ld(w, g, c1, q2) - ld(w, g, c0, q2)) -
                     0.25 * (ld(w, g, p2a, t1) + ld(w, g, p2b, t1) - ld(w, g, m2a, t1) - ld(w, g, m2b, t1));
   const double S1 = ld(w, g, c0, q1) + ld(w, g, c1, q1), S2 = ld(w, g, c0, q2) + ld(w, g, c1, q2);
   // F[3] += 0.5 e (J1 S1 - J2 S2); F[q1] += e J1; F[q2] -= e J2
-  const double bJ1 = e * (0.5 * bF[3] * S1 + bF[q1]);
-  const double bJ2 = e * (-0.5 * bF[3] * S2 - bF[q2]);
-  const double bS1 = 0.5 * e * bF[3] * J1, bS2 = -0.5 * e * bF[3] * J2;
-  auto add = [&](const Cell &c, int m, double v) {
-    atomicAdd(&R[(m - 5) * g.plane + c.o], flip(v, c.mask, m));
+  const long long fp = fplane_of(g, DIR), o = face_idx<DIR>(g, i, j, k);
+  F[o] = e * (0.5 * bF[3] * S1 + bF[q1]);           // bJ1
+  F[fp + o] = 0.5 * e * bF[3] * J1;                 // bS1
+  F[2 * fp + o] = e * (-0.5 * bF[3] * S2 - bF[q2]);  // bJ2
+  F[3 * fp + o] = -0.5 * e * bF[3] * J2;            // bS2
+  F[4 * fp + o] = (0.5 * bF[3] * (J1 * S1 - J2 * S2) + bF[q1] * J1 - bF[q2] * J2) / g.dx;
+}
+
+// The face coordinates f in [flo, n-1] along one axis whose stencil position
+// f + s maps to cell x under cell_at's map (itself, the periodic images, or
+// the mirror images with their negate masks); at most three, held in scalars
+// (selected by index) so that the list stays in registers.
+struct Pre {
+  int f0, f1, f2, cnt;
+  unsigned m0, m1, m2;
+  __device__ __forceinline__ int f(int u) const { return u == 0 ? f0 : (u == 1 ? f1 : f2); }
+  __device__ __forceinline__ unsigned m(int u) const { return u == 0 ? m0 : (u == 1 ? m1 : m2); }
+};
+__device__ __forceinline__ Pre pre_ax(int x, int s, int n, int flo, bool periodic, unsigned slo, unsigned shi) {
+  Pre p{0, 0, 0, 0, 0u, 0u, 0u};
+  auto add = [&](int raw, unsigned m) {
+    const int fc = raw - s;
+    if (fc >= flo && fc <= n - 1) {
+      if (p.cnt == 0) {
+        p.f0 = fc;
+        p.m0 = m;
+      } else if (p.cnt == 1) {
+        p.f1 = fc;
+        p.m1 = m;
+      } else {
+        p.f2 = fc;
+        p.m2 = m;
+      }
+      p.cnt++;
+    }
   };
-  add(c0, q1, bJ1 + bS1);
-  add(c1, q1, -bJ1 + bS1);
-  add(p1a, t1, 0.25 * bJ1);
-  add(p1b, t1, 0.25 * bJ1);
-  add(m1a, t1, -0.25 * bJ1);
-  add(m1b, t1, -0.25 * bJ1);
-  add(c0, q2, -bJ2 + bS2);
-  add(c1, q2, bJ2 + bS2);
-  add(p2a, t1, -0.25 * bJ2);
-  add(p2b, t1, -0.25 * bJ2);
-  add(m2a, t1, 0.25 * bJ2);
-  add(m2b, t1, 0.25 * bJ2);
-  atomicAdd(&geta[ei], (0.5 * bF[3] * (J1 * S1 - J2 * S2) + bF[q1] * J1 - bF[q2] * J2) / g.dx);
+  add(x, 0u);
+  if (periodic) {
+    add(x + n, 0u);
+    add(x - n, 0u);
+  } else {
+    add(-1 - x, slo);
+    add(2 * n - 1 - x, shi);
+  }
+  return p;
+}
+
+// Second half: the B adjoints of cell (i,j,k) from every face along DIR whose
+// stencil reads it, in a fixed order: per role (the face's left cell c0 or
+// right cell c1, each shifted by 0 or +-1 along A1 or A2), every face whose
+// role position maps to (i,j,k), with the sign of the ghost image.
+template <int DIR>
+__device__ __forceinline__ void res_gather(const Geo &g, const double *__restrict__ F, int i, int j, int k,
+                                           double b[3]) {
+  constexpr int q1 = DIR == 0 ? 7 : (DIR == 1 ? 5 : 6);
+  constexpr int q2 = DIR == 0 ? 6 : (DIR == 1 ? 7 : 5);
+  constexpr int t1 = 5 + DIR;
+  constexpr int A1 = DIR == 0 ? 2 : (DIR == 1 ? 0 : 1);
+  constexpr int A2 = DIR == 0 ? 1 : (DIR == 1 ? 2 : 0);
+  const long long fp = fplane_of(g, DIR);
+  const int x[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int dl = 0; dl < 2; dl++) {
+#pragma unroll
+    for (int r = 0; r < 5; r++) {  // 0: no shift; 1, 2: +1, -1 along A1; 3, 4: +1, -1 along A2
+      int s[3] = {0, 0, 0};
+      s[DIR] = dl;
+      if (r == 1 || r == 2) s[A1] = r == 1 ? 1 : -1;
+      if (r == 3 || r == 4) s[A2] = r == 3 ? 1 : -1;
+      Pre P[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const bool per = g.bc[a] == OMHD_BC_PERIODIC;
+        P[a] = pre_ax(x[a], s[a], n[a], (a == DIR && !per) ? -1 : 0, per, g.slo[a], g.shi[a]);
+      }
+      for (int u = 0; u < P[0].cnt; u++)
+        for (int v = 0; v < P[1].cnt; v++)
+          for (int z = 0; z < P[2].cnt; z++) {
+            const long long o = face_idx<DIR>(g, P[0].f(u), P[1].f(v), P[2].f(z));
+            const unsigned m = P[0].m(u) ^ P[1].m(v) ^ P[2].m(z);
+            if (r == 0) {
+              const double bJ1 = F[o], bS1 = F[fp + o], bJ2 = F[2 * fp + o], bS2 = F[3 * fp + o];
+              b[q1 - 5] += flip(dl == 0 ? bJ1 + bS1 : -bJ1 + bS1, m, q1);
+              b[q2 - 5] += flip(dl == 0 ? -bJ2 + bS2 : bJ2 + bS2, m, q2);
+            } else if (r <= 2) {
+              const double bJ1 = F[o];
+              b[t1 - 5] += flip(r == 1 ? 0.25 * bJ1 : -0.25 * bJ1, m, t1);
+            } else {
+              const double bJ2 = F[2 * fp + o];
+              b[t1 - 5] += flip(r == 3 ? -0.25 * bJ2 : 0.25 * bJ2, m, t1);
+            }
+          }
+    }
+  }
 }
 
 // The face-state adjoints of one face.  kind 0: interior/periodic face
@@ -283,19 +375,21 @@ __global__ __launch_bounds__(128) void k_adj_face_lo(const Geo g, const double *
   face_vjp<DIR, LIM>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, 2);
 }
 
-// the resistive flux's reverse at every face along DIR: one thread per cell
-// c (the face (c, c+1), or the high wall face), plus the low wall face for
-// c = 0 at a wall; bF is rebuilt from mu as the face kernels build it
+// the resistive flux's reverse at every face along DIR, first half: one
+// thread per cell c (the face (c, c+1), or the high wall face), plus the low
+// wall face for c = 0 at a wall; bF is rebuilt from mu as the face kernels
+// build it.  rf: the face arrays of all three axes.
 template <int DIR>
 __global__ __launch_bounds__(256) void k_adj_res(const Geo g, const double *__restrict__ w,
                                                  const double *__restrict__ M, double mscale, double mpsi,
-                                                 double *__restrict__ R, double *__restrict__ geta) {
+                                                 double *__restrict__ rf) {
   int i, j, k;
   if (!cell_of(g, i, j, k)) return;
   const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
   const bool wall = g.bc[DIR] != OMHD_BC_PERIODIC;
   const Cell c0 = nbr<DIR>(g, i, j, k, 0), c1 = nbr<DIR>(g, i, j, k, 1);
+  double *F = rf + fbase_of(g, DIR);
   double bF[NV];
   for (int f = 0; f < ((wall && c == 0) ? 2 : 1); f++) {
     const int kind = f == 1 ? 2 : ((wall && c == n - 1) ? 1 : 0);
@@ -305,9 +399,87 @@ __global__ __launch_bounds__(256) void k_adj_res(const Geo g, const double *__re
       const double mu0 = M[m * g.plane + c0.o];
       bF[m] = kind == 0 ? s * (M[m * g.plane + c1.o] - mu0) : (kind == 1 ? -(s * mu0) : s * mu0);
     }
-    if (kind == 2) resistive_vjp<DIR>(g, w, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2), bF, R, geta);
-    else resistive_vjp<DIR>(g, w, i, j, k, bF, R, geta);
+    if (kind == 2) resistive_face<DIR>(g, w, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2), bF, F);
+    else resistive_face<DIR>(g, w, i, j, k, bF, F);
   }
+}
+
+// res_gather for a cell one or more cells from every boundary: each role's
+// position is then read by exactly one face, (i,j,k) minus the role's shift,
+// with no ghost image; the same sums in the same order as res_gather.
+template <int DIR>
+__device__ __forceinline__ void res_gather_inner(const Geo &g, const double *__restrict__ F, int i, int j, int k,
+                                                 double b[3]) {
+  constexpr int q1 = DIR == 0 ? 7 : (DIR == 1 ? 5 : 6);
+  constexpr int q2 = DIR == 0 ? 6 : (DIR == 1 ? 7 : 5);
+  constexpr int t1 = 5 + DIR;
+  constexpr int A1 = DIR == 0 ? 2 : (DIR == 1 ? 0 : 1);
+  constexpr int A2 = DIR == 0 ? 1 : (DIR == 1 ? 2 : 0);
+  const long long fp = fplane_of(g, DIR);
+  const int x[3] = {i, j, k};
+#pragma unroll
+  for (int dl = 0; dl < 2; dl++) {
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+      int s[3] = {0, 0, 0};
+      s[DIR] = dl;
+      if (r == 1 || r == 2) s[A1] = r == 1 ? 1 : -1;
+      if (r == 3 || r == 4) s[A2] = r == 3 ? 1 : -1;
+      const long long o = face_idx<DIR>(g, x[0] - s[0], x[1] - s[1], x[2] - s[2]);
+      if (r == 0) {
+        const double bJ1 = F[o], bS1 = F[fp + o], bJ2 = F[2 * fp + o], bS2 = F[3 * fp + o];
+        b[q1 - 5] += dl == 0 ? bJ1 + bS1 : -bJ1 + bS1;
+        b[q2 - 5] += dl == 0 ? -bJ2 + bS2 : bJ2 + bS2;
+      } else if (r <= 2) {
+        const double bJ1 = F[o];
+        b[t1 - 5] += r == 1 ? 0.25 * bJ1 : -0.25 * bJ1;
+      } else {
+        const double bJ2 = F[2 * fp + o];
+        b[t1 - 5] += r == 3 ? -0.25 * bJ2 : 0.25 * bJ2;
+      }
+    }
+  }
+}
+
+// second half: one thread per cell, the B adjoints from the faces of all
+// three axes, written to R (3 planes, slots bx, by, bz)
+__global__ __launch_bounds__(256) void k_adj_res_gather(const Geo g, const double *__restrict__ rf,
+                                                        double *__restrict__ R) {
+  int i, j, k;
+  if (!cell_of(g, i, j, k)) return;
+  double b[3] = {0.0, 0.0, 0.0};
+  if (i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2) {
+    res_gather_inner<0>(g, rf + fbase_of(g, 0), i, j, k, b);
+    res_gather_inner<1>(g, rf + fbase_of(g, 1), i, j, k, b);
+    res_gather_inner<2>(g, rf + fbase_of(g, 2), i, j, k, b);
+  } else {
+    res_gather<0>(g, rf + fbase_of(g, 0), i, j, k, b);
+    res_gather<1>(g, rf + fbase_of(g, 1), i, j, k, b);
+    res_gather<2>(g, rf + fbase_of(g, 2), i, j, k, b);
+  }
+  const long long o = (long long)k * g.zstride + (long long)j * g.pitch + i;
+#pragma unroll
+  for (int m = 0; m < 3; m++) R[m * g.plane + o] = b[m];
+}
+
+// the eta adjoints: one thread per entry (DIR, i+1, j+1) of geta, the face
+// (i, j) along DIR summed over k in increasing order and added to the entry
+__global__ __launch_bounds__(256) void k_adj_eta(const Geo g, const double *__restrict__ rf,
+                                                 double *__restrict__ geta) {
+  const int ix = g.nx + 2, jx = g.ny + 2;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3LL * ix * jx) return;
+  const int d = (int)(t / ((long long)ix * jx)), r = (int)(t % ((long long)ix * jx));
+  const int fi = r % ix - 1, fj = r / ix - 1;
+  const bool wx = g.bc[0] != OMHD_BC_PERIODIC, wy = g.bc[1] != OMHD_BC_PERIODIC, wz = g.bc[2] != OMHD_BC_PERIODIC;
+  if (fi < ((d == 0 && wx) ? -1 : 0) || fi > g.nx - 1 || fj < ((d == 1 && wy) ? -1 : 0) || fj > g.ny - 1) return;
+  const int fx = g.nx + (d == 0), fy = g.ny + (d == 1);
+  const long long fp = fplane_of(g, d);
+  const double *ge = rf + fbase_of(g, d) + 4 * fp;
+  double s = 0.0;
+  for (int fk = (d == 2 && wz) ? -1 : 0; fk < g.nz; fk++)
+    s += ge[((long long)(fk + (d == 2)) * fy + (fj + (d == 1))) * fx + (fi + (d == 0))];
+  geta[t] += s;
 }
 
 struct Combine {
@@ -397,7 +569,7 @@ __global__ void k_lam_copy(double *__restrict__ dst, const double *__restrict__ 
 }
 
 typedef void (*face_fn)(const Geo, const double *, const double *, double, double, double, double *, double *);
-typedef void (*res_fn)(const Geo, const double *, const double *, double, double, double *, double *);
+typedef void (*res_fn)(const Geo, const double *, const double *, double, double, double *);
 typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, long long, const double *,
                           const Combine);
 
@@ -445,6 +617,7 @@ struct omhd_adjoint {
   double *tape = nullptr;      // 2 * every states: (U^n, U*^n) of one segment
   double *work = nullptr;      // w, AL[3 axes], AR[3 axes], lam, lam*
   double *R = nullptr;         // resistive: B adjoints (3 planes)
+  double *rf = nullptr;        // resistive: face arrays, 5 planes per axis (fplane_of)
   double *geta = nullptr;      // resistive: adjoint of each face's eta, 3 (nx+2)(ny+2)
   std::vector<double> geta_host;
   bool have_forward = false, have_backward = false;
@@ -493,6 +666,10 @@ extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every
   if (e == hipSuccess) e = hipMalloc((void **)&a->work, a->state * 9 * sizeof(double));
   if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->R, (size_t)h->plane * 3 * sizeof(double));
   if (e == hipSuccess && h->cfg.resistive) e = hipMalloc((void **)&a->geta, a->neta() * sizeof(double));
+  if (e == hipSuccess && h->cfg.resistive) {
+    const Geo g = geo_of(h);
+    e = hipMalloc((void **)&a->rf, (size_t)fbase_of(g, 3) * sizeof(double));
+  }
   if (e != hipSuccess) {
     omhd_adjoint_destroy(a);
     return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
@@ -504,7 +681,7 @@ extern "C" int omhd_adjoint_create(omhd_handle *h, int32_t nsteps, int32_t every
 extern "C" int omhd_adjoint_destroy(omhd_adjoint *a) {
   if (!a) return ok();
   if (a->h && a->h->stream) (void)hipStreamSynchronize(a->h->stream);
-  for (double *p : {a->ckpt, a->tape, a->work, a->R, a->geta})
+  for (double *p : {a->ckpt, a->tape, a->work, a->R, a->geta, a->rf})
     if (p) (void)hipFree(p);
   delete a;
   return ok();
@@ -546,7 +723,7 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
   const int lim = h->cfg.lm_type, res = h->cfg.resistive ? 1 : 0;
   const size_t n = (size_t)g.nx * g.ny * g.nz;
   const unsigned g256 = (unsigned)((n + 255) / 256), g128 = (unsigned)((n + 127) / 128);
-  k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w(), res ? a->R : nullptr);
+  k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w());
   OMHD_HIP(hipGetLastError());
   for (int dir = 0; dir < g.nd; dir++) {
     hipLaunchKernelGGL(face_kernel(dir, 0, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch,
@@ -559,10 +736,16 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
       OMHD_HIP(hipGetLastError());
     }
     if (res) {
-      hipLaunchKernelGGL(res_kernel(dir), dim3(g256), dim3(256), 0, h->stream, g, a->w(), M, mscale, mpsi, a->R,
-                         a->geta);
+      hipLaunchKernelGGL(res_kernel(dir), dim3(g256), dim3(256), 0, h->stream, g, a->w(), M, mscale, mpsi, a->rf);
       OMHD_HIP(hipGetLastError());
     }
+  }
+  if (res) {
+    k_adj_res_gather<<<g256, 256, 0, h->stream>>>(g, a->rf, a->R);
+    OMHD_HIP(hipGetLastError());
+    const long long ne = 3LL * (g.nx + 2) * (g.ny + 2);
+    k_adj_eta<<<(unsigned)((ne + 255) / 256), 256, 0, h->stream>>>(g, a->rf, a->geta);
+    OMHD_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(g.nd == 3 ? gather_kernel<3>(lim) : gather_kernel<2>(lim), dim3(g256), dim3(256), 0, h->stream, g,
                      a->w(), a->AL(), a->AR(), (long long)a->state, res ? a->R : nullptr, fin);

@@ -60,17 +60,19 @@ __device__ __forceinline__ double csqrt(double x) {
 // Wave-level lane shifts (64-wide wavefront, DPP wave_shr/wave_shl by 1).
 // from_prev: lane l receives lane l-1 (lane 0 receives 0).
 // from_next: lane l receives lane l+1 (lane 63 receives 0).
+// bound_ctrl set with full row and bank masks: the lane without a source is
+// written 0 by the DPP move itself, so no destination needs clearing first.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double from_prev(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double from_next(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
