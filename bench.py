@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront chunk (0: library default)")
+    ap.add_argument("--dt-mode", default="device", choices=("host", "device"),
+                    help="kh: dt from the reduced CFL speed on the host each step (host) or on the GPU (device)")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="kh", choices=("kh", "ensemble", "3d", "adjoint"),
@@ -468,6 +470,8 @@ def main():
     s = Solver2D(n, n, dx, cfl=0.4, lm_type=1, time_type=0, bc=("periodic", "wall"), rows_per_wave=a.rows)
     s.upload(U)
     del U
+    if a.dt_mode == "device":
+        s.set_mode(device_dt=True)
     log(f"warmup {a.warmup} steps")
     s.step(a.warmup)
     s.set_timing(True)
@@ -504,7 +508,7 @@ def main():
         "data": "synthetic (seeded KH initial state generated on the host)",
         "config": {"workload": f"2D Kelvin-Helmholtz {n}x{n} FP64, x periodic, y walls, minmod + HLLD + GLM + "
                                f"TVD-RK2, CFL 0.4", "nx": n, "ny": n, "limiter": "minmod", "flux": "HLLD",
-                   "time": "TVD-RK2", "parallelism": "single GPU, one domain"},
+                   "time": "TVD-RK2", "dt": a.dt_mode, "parallelism": "single GPU, one domain"},
         "hbm_frac_step": zups * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": f"k_march stage {dom + 1} (k_march<LIM,{dom + 1},TT,0,0>)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -212,6 +212,22 @@ int omhd_destroy(omhd_handle *h);
 int omhd_set_timing(omhd_handle *h, int32_t enable);
 int omhd_get_timing(omhd_handle *h, double *stage_ms, int64_t *launches);
 
+/* dt mode of the resident step.  The reference computes each step's dt on
+ * the host from the reduced CFL speed (main.f90:80, set_dt.f90:21-65), and
+ * omhd_step does the same by default (host dt mode: one host round trip per
+ * step).  In device dt mode (device_dt = 1) each step's dt, CFL speed, dt/dx
+ * and GLM factor are computed on the GPU after its second stage, the GLM
+ * factor looked up in a table of the host's exp values, so omhd_step queues
+ * all its steps and synchronises once at the end; use_graph = 1 also replays
+ * one captured hipGraph per step (per-stage timing is then not recorded).
+ * Results, t, dt/ch traces, *done and every stop (positivity, dt too small)
+ * are those of host dt mode bit for bit; after a stop, later omhd_step calls
+ * return the same stop until a new state is loaded (omhd_upload,
+ * omhd_set_state, omhd_step_fixed).  Ideal MHD only: a handle with
+ * resistivity is refused (a resistivity-limited dt puts the GLM argument
+ * outside the table).  Needs a state (omhd_upload first). */
+int omhd_set_mode(omhd_handle *h, int32_t device_dt, int32_t use_graph);
+
 /* ---- ensemble of independent 2D grids ------------------------------------ */
 /* No counterpart routine in the reference (each reference run is one grid):
  * n independent 2D grids, each with its own extents, boundaries and CFL dt,

@@ -89,6 +89,7 @@ def _load() -> C.CDLL:
         "omhd_state_layout": [C.c_void_p, C.POINTER(DP), C.POINTER(C.c_int64), C.POINTER(I32)],
         "omhd_destroy": [C.c_void_p],
         "omhd_set_timing": [C.c_void_p, I32],
+        "omhd_set_mode": [C.c_void_p, I32, I32],
         "omhd_batch_create": [C.POINTER(Config), I32, C.POINTER(C.c_void_p)],
         "omhd_batch_upload": [C.c_void_p, I32, DP],
         "omhd_batch_download": [C.c_void_p, I32, DP, DP],

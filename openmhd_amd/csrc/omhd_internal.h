@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/omhd.h"
 
 namespace omhd {
@@ -38,6 +40,21 @@ struct omhd_handle {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double stage_ms[2] = {0.0, 0.0};
   long long stage_launches[2] = {0, 0};
+  // device dt mode (omhd_set_mode): each step's dt, CFL speed and GLM factor
+  // are computed on the device from the stage-2 reduction, so a call of n
+  // steps needs no host round trip until its end
+  bool device_dt = false, use_graph = false;
+  void *solo = nullptr;       // device block: step parameters, time, step count, GLM table
+  void *solo_host = nullptr;  // pinned mirror of it
+  double *trace_d = nullptr;  // device dt/ch trace, 2 doubles per step
+  long long trace_cap = 0;
+  hipGraph_t graph = nullptr;  // one captured step (stages + the dt kernel)
+  hipGraphExec_t gexec = nullptr;
+  bool graph_traced = false;            // whether the captured step writes the trace
+  std::vector<hipEvent_t> step_ev;      // per-step timing events of device dt mode
+  int halt_rc = 0;                      // a stop found on the device, returned again by later calls
+  int halt_err[4] = {0, 0, 0, 0};
+  double halt_dt = 0.0;
 };
 
 
@@ -46,9 +63,11 @@ namespace omhd {
 // One RK stage of the resident step with explicit state pointers (the handle's
 // own buffers are not touched): stage 1 reads `us` (psi scaled by glm_f on
 // load) and writes U* to `out`; stage 2 reads U* = `us` and U^n = `u0` and
-// writes U^{n+1} to `out`.  dt and ch are the step's (frozen) values.  Launched
-// on the handle's stream; returns an OMHD status.
-int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, double *out, double dt, double ch);
+// writes U^{n+1} to `out`.  dt and ch are the step's (frozen) values, unless
+// dpar names device-side step parameters (device dt mode), which the kernels
+// read instead.  Launched on the handle's stream; returns an OMHD status.
+int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, double *out, double dt, double ch,
+                 const void *dpar = nullptr);
 
 // The handle's CFL speed from its current state (the upload pass).
 int refresh_cfl(omhd_handle *h);

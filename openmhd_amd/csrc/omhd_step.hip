@@ -181,6 +181,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
   double ch = p.ch, ch2 = p.ch2, dtx = p.dtx, glm_f = p.glm_f;
   if (p.dpar) {
     const DevPar *dp = (const DevPar *)p.dpar;
+    if (!(dp->dt > 0.0)) return;  // device dt mode halted by a stop: the state stays as it was
     ch = dp->ch;
     ch2 = dp->ch2;
     dtx = STAGE == 1 ? dp->dtx1 : dp->dtx2;
@@ -574,6 +575,74 @@ __global__ void k_device_dt(DevPar *par, unsigned long long *vmax_bits, const do
   vmax_bits[i] = 0ull;
 }
 
+// Device dt mode of one handle (omhd_set_mode).  `par` holds the parameters
+// of the step about to run (dt == 0: halted by a stop); t and done are the
+// time and the count of completed steps of the current call; bad_dt is a dt
+// that failed set_dt's test.
+struct SoloDev {
+  DevPar par;
+  double t;
+  long long done;
+  double bad_dt;
+  long long glm_x0;
+  double glm_tab[GLM_TAB];
+};
+
+__global__ void k_solo_begin(SoloDev *s, unsigned long long *red, double t0) {
+  s->t = t0;
+  s->done = 0;
+  red[0] = 0ull;
+}
+
+// After the stages of a step (one thread): the bookkeeping omhd_step does on
+// the host, in the same order and arithmetic.  A failure inside the step
+// (err) leaves t and done as they were; a failure of the completed step's
+// output (next: the following u2v or set_dt would stop) counts the step; a
+// dt below set_dt's bound (set_dt.f90:55-65) stops before the next step.
+// Any stop sets dt = 0, after which the stage kernels and this kernel return
+// at once.  Otherwise the next step's parameters come from the reduced CFL
+// speed: dt = (cfl*dx)/ch, dt/dx per stage, and the GLM factor from the table
+// of host exp values around x0 (see k_device_dt).
+__global__ void k_solo_dt(SoloDev *s, unsigned long long *red, double cfl, double dx, int tt, double *trace) {
+  const DevPar q = s->par;
+  if (!(q.dt > 0.0)) return;
+  int *err = (int *)(red + 1);
+  const int *nxt = (const int *)(red + 3);
+  if (err[0] != 0) {
+    s->par.dt = 0.0;
+    return;
+  }
+  s->t += q.dt;
+  if (trace) {
+    trace[2 * s->done] = q.dt;
+    trace[2 * s->done + 1] = q.ch;
+  }
+  s->done += 1;
+  if (nxt[0] != 0 || nxt[1] != 0) {
+    s->par.dt = 0.0;
+    return;
+  }
+  const double vmax = __longlong_as_double((long long)red[0]);
+  red[0] = 0ull;
+  const double dt = cfl * dx / vmax;
+  if (!(dt >= 1.0e-7)) {
+    s->bad_dt = dt;
+    s->par.dt = 0.0;
+    record_error(err, OMHD_ERR_DT, 0, 0, 0);
+    return;
+  }
+  DevPar n;
+  n.ch = vmax;
+  n.ch2 = vmax * vmax;
+  n.dtx1 = tt == 0 ? dt / dx : 0.5 * dt / dx;
+  n.dtx2 = dt / dx;
+  const double x = -0.5 * dt * vmax / 0.2;
+  const long long off = (long long)__double_as_longlong(x) - s->glm_x0;
+  n.glm_f = (off >= -GLM_TAB_HALF && off <= GLM_TAB_HALF) ? s->glm_tab[off + GLM_TAB_HALF] : exp(x);
+  n.dt = dt;
+  s->par = n;
+}
+
 // CFL speed of the resident state (set_dt.f90:21-45), used once after upload
 template <int ND>
 __global__ void k_cfl(const double *__restrict__ s, long long plane, long long zstride, int pitch, int nx, int ny,
@@ -851,6 +920,8 @@ extern "C" int omhd_set_resistivity(omhd_handle *h, double eta0, double eta1, do
   return ok();
 }
 
+static void solo_drop_graph(omhd_handle *h);
+
 extern "C" int omhd_destroy(omhd_handle *h) {
   if (!h) return ok();
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -862,6 +933,11 @@ extern "C" int omhd_destroy(omhd_handle *h) {
   if (h->red_host) (void)hipHostFree(h->red_host);
   for (int i = 0; i < 3; i++)
     if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+  for (hipEvent_t e : h->step_ev) (void)hipEventDestroy(e);
+  solo_drop_graph(h);
+  if (h->solo) (void)hipFree(h->solo);
+  if (h->solo_host) (void)hipHostFree(h->solo_host);
+  if (h->trace_d) (void)hipFree(h->trace_d);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return ok();
@@ -942,6 +1018,49 @@ static int read_error(omhd_handle *h, const int err[4]) {
   return positivity_error(h, (err[0] & ERR_IN_USTAR) ? h->U1 : h->U, e);
 }
 
+static void solo_drop_graph(omhd_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
+// Device dt mode: the parameters of the next step from the current CFL speed,
+// exactly as step_dt and stage_launch compute them on the host, and the table
+// of host exp values around x0 = (-0.5*(cfl*dx))/0.2 for the device's GLM
+// factors.  A dt below set_dt's bound halts the handle: its next step call
+// returns that stop, as omhd_step's would.
+static int solo_seed(omhd_handle *h) {
+  const omhd_config &c = h->cfg;
+  SoloDev *sh = (SoloDev *)h->solo_host;
+  memset(sh, 0, sizeof(SoloDev));
+  const double x0 = -0.5 * (c.cfl * c.dx) / 0.2;
+  memcpy(&sh->glm_x0, &x0, sizeof x0);
+  for (int j = -GLM_TAB_HALF; j <= GLM_TAB_HALF; j++) {
+    const long long xb = sh->glm_x0 + j;
+    double x;
+    memcpy(&x, &xb, sizeof x);
+    sh->glm_tab[j + GLM_TAB_HALF] = exp(x);
+  }
+  h->halt_rc = 0;
+  const double ch = h->vmax, dt = c.cfl * c.dx / ch;
+  if (!(dt >= 1.0e-7)) {
+    sh->par.dt = 0.0;
+    h->halt_rc = OMHD_ERR_DT;
+    h->halt_dt = dt;
+  } else {
+    sh->par.ch = ch;
+    sh->par.ch2 = ch * ch;
+    sh->par.dtx1 = c.time_type == 0 ? dt / c.dx : 0.5 * dt / c.dx;
+    sh->par.dtx2 = dt / c.dx;
+    sh->par.glm_f = exp(-0.5 * dt * ch / 0.2);  // glm_ss2.f90:27
+    sh->par.dt = dt;
+  }
+  OMHD_HIP(hipMemcpyAsync(h->solo, sh, sizeof(SoloDev), hipMemcpyHostToDevice, h->stream));
+  OMHD_HIP(hipStreamSynchronize(h->stream));
+  return OMHD_OK;
+}
+
 static int cfl_pass(omhd_handle *h) {
   const omhd_config &c = h->cfg;
   k_zero_red<<<1, 64, 0, h->stream>>>(h->red);
@@ -959,7 +1078,7 @@ static int cfl_pass(omhd_handle *h) {
   if (int rc = read_red(h, vm, err)) return rc;
   if (err[0]) return read_error(h, err);
   h->vmax = vm;
-  return OMHD_OK;
+  return h->device_dt ? solo_seed(h) : OMHD_OK;
 }
 
 extern "C" int omhd_upload(omhd_handle *h, const double *U) {
@@ -1030,7 +1149,8 @@ static unsigned grid_of(const Sweep &p) { return (unsigned)((p.ntiles + 7) / 8 *
 
 namespace omhd {
 
-int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, double *out, double dt, double ch) {
+int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, double *out, double dt, double ch,
+                 const void *dpar) {
   const omhd_config &c = h->cfg;
   const bool three = h->nd == 3;
   const int res = c.resistive ? 1 : 0;
@@ -1051,6 +1171,7 @@ int stage_launch(omhd_handle *h, int stage, const double *us, const double *u0, 
     q->out = out;
     q->dtx = dtx;
     q->ftype = c.flux_type;
+    q->dpar = (const double *)dpar;
   }
   hipLaunchKernelGGL(a, dim3(grid_of(pa)), dim3(64), 0, h->stream, pa);
   OMHD_HIP(hipGetLastError());
@@ -1078,10 +1199,156 @@ static int step_dt(omhd_handle *h, double &dt, double &ch) {
   return OMHD_OK;
 }
 
+// one step of device dt mode: the two stages read their parameters from the
+// device block, then k_solo_dt prepares the next step's
+static int solo_step_launch(omhd_handle *h, double *trace) {
+  SoloDev *sd = (SoloDev *)h->solo;
+  if (int rc = stage_launch(h, 1, h->U, h->U, h->U1, 0.0, 0.0, &sd->par)) return rc;
+  if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, 0.0, 0.0, &sd->par)) return rc;
+  k_solo_dt<<<1, 1, 0, h->stream>>>(sd, h->red, h->cfg.cfl, h->cfg.dx, h->cfg.time_type, trace);
+  OMHD_HIP(hipGetLastError());
+  return OMHD_OK;
+}
+
+// the stop a halted handle returns again
+static int solo_halt_error(omhd_handle *h) {
+  if (h->halt_rc == OMHD_ERR_DT) return set_error(OMHD_ERR_DT, "dt is too small", 0, 0, 0, h->halt_dt);
+  if (h->halt_rc == OMHD_ERR_PRESSURE || h->halt_rc == OMHD_ERR_DENSITY) return positivity_error(h, h->U, h->halt_err);
+  return state_error(h->halt_err);
+}
+
+// omhd_step in device dt mode: the steps are queued without waiting, and the
+// time, traces, step count and any stop are read once at the end.  The
+// results are those of the host-dt loop below bit for bit, the stop included.
+static int step_device(omhd_handle *h, int nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
+  if (h->halt_rc) return solo_halt_error(h);
+  const bool tr = dt_trace || ch_trace;
+  if (tr && h->trace_cap < nsteps) {
+    if (h->trace_d) OMHD_HIP(hipFree(h->trace_d));
+    h->trace_d = nullptr;
+    h->trace_cap = 0;
+    OMHD_HIP(hipMalloc((void **)&h->trace_d, 2 * (size_t)nsteps * sizeof(double)));
+    h->trace_cap = nsteps;
+    solo_drop_graph(h);  // the captured step names the trace buffer
+  }
+  double *trace = tr ? h->trace_d : nullptr;
+  if (h->gexec && h->graph_traced != tr) solo_drop_graph(h);
+  if (h->use_graph && !h->gexec && nsteps > 0) {
+    OMHD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = solo_step_launch(h, trace);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(OMHD_ERR_HIP, hipGetErrorString(e));
+    h->graph = g;
+    OMHD_HIP(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->graph_traced = tr;
+  }
+  const bool timed = h->timing && !h->use_graph;
+  if (timed)
+    while (h->step_ev.size() < 3 * (size_t)nsteps) {
+      hipEvent_t e;
+      OMHD_HIP(hipEventCreate(&e));
+      h->step_ev.push_back(e);
+    }
+  SoloDev *sd = (SoloDev *)h->solo;
+  k_solo_begin<<<1, 1, 0, h->stream>>>(sd, h->red, t ? *t : 0.0);
+  OMHD_HIP(hipGetLastError());
+  for (int n = 0; n < nsteps; n++) {
+    if (h->use_graph) {
+      OMHD_HIP(hipGraphLaunch(h->gexec, h->stream));
+      continue;
+    }
+    if (!timed) {
+      if (int rc = solo_step_launch(h, trace)) return rc;
+      continue;
+    }
+    OMHD_HIP(hipEventRecord(h->step_ev[3 * n], h->stream));
+    if (int rc = stage_launch(h, 1, h->U, h->U, h->U1, 0.0, 0.0, &sd->par)) return rc;
+    OMHD_HIP(hipEventRecord(h->step_ev[3 * n + 1], h->stream));
+    if (int rc = stage_launch(h, 2, h->U1, h->U, h->U, 0.0, 0.0, &sd->par)) return rc;
+    OMHD_HIP(hipEventRecord(h->step_ev[3 * n + 2], h->stream));
+    k_solo_dt<<<1, 1, 0, h->stream>>>(sd, h->red, h->cfg.cfl, h->cfg.dx, h->cfg.time_type, trace);
+    OMHD_HIP(hipGetLastError());
+  }
+  SoloDev *sh = (SoloDev *)h->solo_host;
+  OMHD_HIP(hipMemcpyAsync(sh, sd, sizeof(SoloDev), hipMemcpyDeviceToHost, h->stream));
+  OMHD_HIP(hipMemcpyAsync(h->red_host, h->red, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> trh(tr ? 2 * (size_t)nsteps : 0);
+  if (tr && nsteps > 0)
+    OMHD_HIP(hipMemcpyAsync(trh.data(), h->trace_d, trh.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  OMHD_HIP(hipStreamSynchronize(h->stream));
+  const int nd = (int)sh->done;
+  if (t) *t = sh->t;
+  if (done) *done = nd;
+  for (int n = 0; n < nd; n++) {
+    if (dt_trace) dt_trace[n] = trh[2 * n];
+    if (ch_trace) ch_trace[n] = trh[2 * n + 1];
+  }
+  if (timed)
+    for (int n = 0; n < nsteps; n++) {
+      float a = 0.f, b = 0.f;
+      OMHD_HIP(hipEventElapsedTime(&a, h->step_ev[3 * n], h->step_ev[3 * n + 1]));
+      OMHD_HIP(hipEventElapsedTime(&b, h->step_ev[3 * n + 1], h->step_ev[3 * n + 2]));
+      h->stage_ms[0] += a;
+      h->stage_ms[1] += b;
+      h->stage_launches[0]++;
+      h->stage_launches[1]++;
+    }
+  if (sh->par.dt > 0.0) h->vmax = sh->par.ch;
+  int err[4], next[2];
+  memcpy(err, &h->red_host[1], sizeof err);
+  memcpy(next, &h->red_host[3], sizeof next);
+  if (err[0] == OMHD_ERR_DT) {  // the next step's dt: omhd_step stops there, at the start of that step
+    h->halt_rc = OMHD_ERR_DT;
+    h->halt_dt = sh->bad_dt;
+    return nd == nsteps ? ok() : solo_halt_error(h);
+  }
+  if (err[0]) {
+    h->halt_rc = err[0] & ~ERR_IN_USTAR;
+    for (int i = 0; i < 4; i++) h->halt_err[i] = err[i];
+    h->halt_err[0] = h->halt_rc;
+    return read_error(h, err);
+  }
+  if (next[0] || next[1]) {
+    h->halt_rc = next[0] ? next[0] : next[1];
+    h->halt_err[0] = h->halt_rc;
+    h->halt_err[1] = h->halt_err[2] = h->halt_err[3] = 0;
+    return solo_halt_error(h);
+  }
+  return ok();
+}
+
+extern "C" int omhd_set_mode(omhd_handle *h, int32_t device_dt, int32_t use_graph) {
+  if (!h) return set_error(OMHD_ERR_ARG, "null handle");
+  if (use_graph && !device_dt) return set_error(OMHD_ERR_ARG, "graph capture needs device dt mode");
+  if (device_dt && h->cfg.resistive)
+    return set_error(OMHD_ERR_ARG,
+                     "device dt mode is built for the ideal step: a resistivity-limited dt moves the GLM factor's "
+                     "argument off the table of host exp values");
+  if (device_dt && !h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
+  if (device_dt && !h->solo) {
+    OMHD_HIP(hipMalloc(&h->solo, sizeof(SoloDev)));
+    OMHD_HIP(hipHostMalloc(&h->solo_host, sizeof(SoloDev)));
+  }
+  if (!use_graph || !device_dt) solo_drop_graph(h);
+  const bool enter = device_dt && !h->device_dt;
+  h->device_dt = device_dt != 0;
+  h->use_graph = use_graph != 0;
+  if (enter)
+    if (int rc = solo_seed(h)) return rc;
+  if (!device_dt) h->halt_rc = 0;
+  return ok();
+}
+
 extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
   if (done) *done = 0;
   if (!h) return set_error(OMHD_ERR_ARG, "null handle");
   if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
+  if (h->device_dt) return step_device(h, nsteps, t, dt_trace, ch_trace, done);
   for (int n = 0; n < nsteps; n++) {
     double dt, ch;
     if (int rc = step_dt(h, dt, ch)) return rc;

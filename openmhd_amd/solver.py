@@ -151,6 +151,15 @@ class Solver:
     def set_timing(self, enable=True):
         check(lib.omhd_set_timing(self.h, 1 if enable else 0), "omhd_set_timing")
 
+    def set_mode(self, device_dt=False, graph=False):
+        """device_dt=True: each step's dt, CFL speed and GLM factor are computed
+        on the GPU after its second stage (the GLM factor from a table of the
+        host's exp values), so step(n) queues all n steps and waits once;
+        graph=True also replays one captured hipGraph per step (no per-stage
+        timing then).  Results, t, traces and stops equal the default host-dt
+        mode bit for bit.  Ideal MHD only; needs a state."""
+        check(lib.omhd_set_mode(self.h, 1 if device_dt else 0, 1 if graph else 0), "omhd_set_mode")
+
     def timing(self):
         """(stage_ms[2], launches[2]) accumulated since set_timing(True)."""
         ms = (C.c_double * 2)()
