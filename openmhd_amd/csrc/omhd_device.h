@@ -40,20 +40,22 @@ __device__ __forceinline__ double dmin(double a, double b) { return __builtin_fm
 // factors are 2^0 and the select is not taken, so there the core below (the
 // same operations on the same operands, in the same order) gives the same
 // result with 8 fewer instructions.  Every other argument (zero, denormals,
-// inf, NaN, negatives) goes to __builtin_sqrt.
+// inf, NaN, negatives) takes __builtin_sqrt.  The core runs on every lane and
+// the test is one branch for the whole wavefront (taken only when some lane
+// is out of range, then a per-lane select), not an if/else on the exec mask.
 __device__ __forceinline__ double csqrt(double x) {
-  if (__builtin_expect(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023, 1)) {
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = y * 0.5;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
-  }
-  return __builtin_sqrt(x);
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  const bool in = x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023;
+  if (__builtin_expect(__any(!in), 0)) return in ? g : __builtin_sqrt(x);
+  return g;
 }
 
 // ---------------------------------------------------------------------------

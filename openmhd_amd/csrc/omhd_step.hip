@@ -141,10 +141,14 @@ __device__ __forceinline__ double &lane_at(double *a, long long plane, int m, lo
   return *(double *)((char *)(a + (m * plane + ro)) + off8);
 }
 
-__device__ __forceinline__ void put66(double *row, int lane, double v) {
+// Store this lane's value of a 66-entry LDS row: entry lane + 1, and again
+// at `edge`, the lane's second entry (0 for lane 0, 65 for lane 63, lane + 1
+// itself for the others).  Two plain stores per lane instead of lane tests
+// and divergent branches.
+__device__ __forceinline__ int edge66(int lane) { return lane == 0 ? 0 : (lane == 63 ? 65 : lane + 1); }
+__device__ __forceinline__ void put66(double *row, int lane, int edge, double v) {
   row[lane + 1] = v;
-  if (lane == 0) row[0] = v;
-  if (lane == 63) row[65] = v;
+  row[edge] = v;
 }
 
 template <int RES, int PASS>
@@ -163,6 +167,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
   constexpr int MD = PASS == PASS_Z ? 2 : 1;  // march axis = direction of its HLLD
   constexpr int ND = PASS == PASS_2D ? 2 : 3;
   const int lane = threadIdx.x;
+  const int edge = edge66(lane);
   const int strip = t % p.nstrip;
   const int r2 = t / p.nstrip;
   const int chunk = r2 % p.nchunk, slice = r2 / p.nchunk;  // slice: k (XY) or j (Z)
@@ -247,7 +252,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         unsigned ymask;
         map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ymask);
         if constexpr (PASS == PASS_XY) {
-          put66(lds.bxm, lane, lds.ring[sc][5][lane + 1]);  // Bx of row rn-3, before its slot is reused
+          put66(lds.bxm, lane, edge, lds.ring[sc][5][lane + 1]);  // Bx of row rn-3, before its slot is reused
 #pragma unroll
           for (int s2 = 0; s2 < 2; s2++) {
             int km;
@@ -255,8 +260,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zmask);
             const unsigned mk = xmask ^ ymask ^ zmask;
             const long long ro = (long long)km * p.zstride + (long long)rm * p.pitch;
-            put66(lds.aux[sc][s2], lane, flip(lane_at(p.us, plane, 5, ro, cm8), mk, 5));
-            put66(lds.aux[sc][2 + s2], lane, flip(lane_at(p.us, plane, 6, ro, cm8), mk, 6));
+            put66(lds.aux[sc][s2], lane, edge, flip(lane_at(p.us, plane, 5, ro, cm8), mk, 5));
+            put66(lds.aux[sc][2 + s2], lane, edge, flip(lane_at(p.us, plane, 6, ro, cm8), mk, 6));
           }
         } else {
 #pragma unroll
@@ -266,12 +271,12 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             map_axis(slice + (s2 ? 1 : -1), p.ny, p.bc[1], p.slo[1], p.shi[1], jm, jmask);
             const unsigned mk = xmask ^ ymask ^ jmask;
             const long long ro = (long long)rm * p.zstride + (long long)jm * p.pitch;
-            put66(lds.aux[sc][s2], lane, flip(lane_at(p.us, plane, 7, ro, cm8), mk, 7));
+            put66(lds.aux[sc][s2], lane, edge, flip(lane_at(p.us, plane, 7, ro, cm8), mk, 7));
           }
         }
       }
 #pragma unroll
-      for (int m = 0; m < NV; m++) put66(lds.ring[sc][m], lane, W[m]);
+      for (int m = 0; m < NV; m++) put66(lds.ring[sc][m], lane, edge, W[m]);
     }
     __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
     // stage 2: issue the loads of U^n for the row combined at the end of this
