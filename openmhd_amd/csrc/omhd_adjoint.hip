@@ -94,18 +94,14 @@ __device__ __forceinline__ bool cell_of_tiled(const Geo &g, int &i, int &j, int 
   return true;
 }
 
+// by selects, not branches: the lane-dependent tests would otherwise split
+// the wavefront at every stencil position
 __device__ __forceinline__ int map_ax(int c, int n, int bc, unsigned slo, unsigned shi, unsigned &mask) {
-  if (c < 0) {
-    if (bc == OMHD_BC_PERIODIC) return c + n;
-    mask ^= slo;
-    return -1 - c;
-  }
-  if (c >= n) {
-    if (bc == OMHD_BC_PERIODIC) return c - n;
-    mask ^= shi;
-    return 2 * n - 1 - c;
-  }
-  return c;
+  const bool per = bc == OMHD_BC_PERIODIC, lo = c < 0, hi = c >= n;
+  mask ^= (lo && !per) ? slo : 0u;
+  mask ^= (hi && !per) ? shi : 0u;
+  const int rlo = per ? c + n : -1 - c, rhi = per ? c - n : 2 * n - 1 - c;
+  return lo ? rlo : (hi ? rhi : c);
 }
 
 // interior cell of (i, j, k) (each coordinate at most two cells outside) and
@@ -114,7 +110,17 @@ struct Cell {
   long long o;
   unsigned mask;
 };
+// MIR = false: every axis is periodic (the host's choice of kernel), so the
+// map is a wrap by selects and the mask is the constant 0 (no sign flips).
+__device__ __forceinline__ int wrap_ax(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+template <bool MIR = true>
 __device__ __forceinline__ Cell cell_at(const Geo &g, int i, int j, int k) {
+  if constexpr (!MIR) {
+    i = wrap_ax(i, g.nx);
+    j = wrap_ax(j, g.ny);
+    if (g.nd == 3) k = wrap_ax(k, g.nz);
+    return Cell{(long long)k * g.zstride + (long long)j * g.pitch + i, 0u};
+  }
   unsigned mk = 0;
   i = map_ax(i, g.nx, g.bc[0], g.slo[0], g.shi[0], mk);
   j = map_ax(j, g.ny, g.bc[1], g.slo[1], g.shi[1], mk);
@@ -122,9 +128,9 @@ __device__ __forceinline__ Cell cell_at(const Geo &g, int i, int j, int k) {
   return Cell{(long long)k * g.zstride + (long long)j * g.pitch + i, mk};
 }
 
-template <int DIR>
+template <int DIR, bool MIR = true>
 __device__ __forceinline__ Cell nbr(const Geo &g, int i, int j, int k, int s) {
-  return cell_at(g, i + (DIR == 0 ? s : 0), j + (DIR == 1 ? s : 0), k + (DIR == 2 ? s : 0));
+  return cell_at<MIR>(g, i + (DIR == 0 ? s : 0), j + (DIR == 1 ? s : 0), k + (DIR == 2 ? s : 0));
 }
 
 __device__ __forceinline__ double flip(double v, unsigned mask, int m) {
@@ -303,11 +309,11 @@ __device__ __forceinline__ void res_gather(const Geo &g, const double *__restric
 // mscale * M (slot 8 also times mpsi).  Stored by owning cell: AL(c) for the
 // left state of (c, c+1), AR(c) for the right state of (c-1, c); a wall
 // face's two states both belong to its interior cell.
-template <int DIR, int LIM>
+template <int DIR, int LIM, bool MIR = true>
 __device__ __forceinline__ void face_vjp(const Geo &g, const double *__restrict__ w, const double *__restrict__ M,
                                          double mscale, double mpsi, double ch, double *__restrict__ AL,
                                          double *__restrict__ AR, int i, int j, int k, int kind) {
-  const Cell cm = nbr<DIR>(g, i, j, k, -1), c0 = nbr<DIR>(g, i, j, k, 0), c1 = nbr<DIR>(g, i, j, k, 1);
+  const Cell cm = nbr<DIR, MIR>(g, i, j, k, -1), c0 = nbr<DIR, MIR>(g, i, j, k, 0), c1 = nbr<DIR, MIR>(g, i, j, k, 1);
   double VL[NV], VR[NV], bF[NV], bL[NV], bR[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
@@ -317,7 +323,7 @@ __device__ __forceinline__ void face_vjp(const Geo &g, const double *__restrict_
     const double s = m == 8 ? mscale * mpsi : mscale;
     const double mu0 = M[m * g.plane + c0.o];
     if (kind == 0) {
-      const Cell c2 = nbr<DIR>(g, i, j, k, 2);
+      const Cell c2 = nbr<DIR, MIR>(g, i, j, k, 2);
       double hR, hL;
       adj::lim_fwd<double>(LIM, a0, a1, ld(w, g, c2, m), hR, hL);
       VL[m] = gL;
@@ -349,8 +355,9 @@ __device__ __forceinline__ void face_vjp(const Geo &g, const double *__restrict_
   }
 }
 
-// one thread per cell c along DIR: the face (c, c+1), or the high wall face
-template <int DIR, int LIM>
+// one thread per cell c along DIR: the face (c, c+1), or the high wall face.
+// MIR = false: a grid without wall axes (only interior/periodic faces).
+template <int DIR, int LIM, bool MIR>
 __global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__restrict__ w,
                                                   const double *__restrict__ M, double mscale, double mpsi, double ch,
                                                   double *__restrict__ AL, double *__restrict__ AR) {
@@ -358,8 +365,8 @@ __global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__r
   if (!cell_of(g, i, j, k)) return;
   const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
-  const bool high = g.bc[DIR] != OMHD_BC_PERIODIC && c == n - 1;
-  face_vjp<DIR, LIM>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, high ? 1 : 0);
+  const bool high = MIR && g.bc[DIR] != OMHD_BC_PERIODIC && c == n - 1;
+  face_vjp<DIR, LIM, MIR>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, high ? 1 : 0);
 }
 
 // walls only: one thread per cell of the plane c = 0, the low wall face
@@ -496,7 +503,7 @@ struct Combine {
 // q = c + s; at a wall, the edge cells q = 0 and q = n - 1 also read it
 // through their mirror ghost), then the resistive B adjoints, the u2v
 // reverse and the RK/GLM combine into c.out.  AL/AR hold one array per axis.
-template <int LIM, int ND>
+template <int LIM, int ND, bool MIR>
 __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *__restrict__ w,
                                                     const double *__restrict__ AL, const double *__restrict__ AR,
                                                     long long astride, const double *__restrict__ R, const Combine c) {
@@ -513,11 +520,11 @@ __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *_
   for (int d = 0; d < ND; d++) {
     const int cc = d == 0 ? i : (d == 1 ? j : k);
     const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-    const bool wall = g.bc[d] != OMHD_BC_PERIODIC;
+    const bool wall = MIR && g.bc[d] != OMHD_BC_PERIODIC;
     Cell o[5];
 #pragma unroll
     for (int s = 0; s < 5; s++)
-      o[s] = cell_at(g, i + (d == 0 ? s - 2 : 0), j + (d == 1 ? s - 2 : 0), k + (d == 2 ? s - 2 : 0));
+      o[s] = cell_at<MIR>(g, i + (d == 0 ? s - 2 : 0), j + (d == 1 ? s - 2 : 0), k + (d == 2 ? s - 2 : 0));
     const double *al = AL + d * astride, *ar = AR + d * astride;
 #pragma unroll
     for (int m = 0; m < NV; m++) {
@@ -573,34 +580,47 @@ typedef void (*res_fn)(const Geo, const double *, const double *, double, double
 typedef void (*gather_fn)(const Geo, const double *, const double *, const double *, long long, const double *,
                           const Combine);
 
-template <int DIR, int LO>
+// LO: the low wall faces; otherwise MIR selects the kernel for a grid with
+// wall axes (true) or a fully periodic one (false)
+template <int DIR, int LO, bool MIR>
 face_fn face_kernel(int lim) {
   switch (lim) {
-    case 0: return LO ? k_adj_face_lo<DIR, 0> : k_adj_face<DIR, 0>;
-    case 1: return LO ? k_adj_face_lo<DIR, 1> : k_adj_face<DIR, 1>;
-    case 2: return LO ? k_adj_face_lo<DIR, 2> : k_adj_face<DIR, 2>;
-    case 3: return LO ? k_adj_face_lo<DIR, 3> : k_adj_face<DIR, 3>;
-    default: return LO ? k_adj_face_lo<DIR, 4> : k_adj_face<DIR, 4>;
+    case 0: return LO ? k_adj_face_lo<DIR, 0> : k_adj_face<DIR, 0, MIR>;
+    case 1: return LO ? k_adj_face_lo<DIR, 1> : k_adj_face<DIR, 1, MIR>;
+    case 2: return LO ? k_adj_face_lo<DIR, 2> : k_adj_face<DIR, 2, MIR>;
+    case 3: return LO ? k_adj_face_lo<DIR, 3> : k_adj_face<DIR, 3, MIR>;
+    default: return LO ? k_adj_face_lo<DIR, 4> : k_adj_face<DIR, 4, MIR>;
   }
 }
 
-face_fn face_kernel(int dir, int lo, int lim) {
-  if (dir == 0) return lo ? face_kernel<0, 1>(lim) : face_kernel<0, 0>(lim);
-  if (dir == 1) return lo ? face_kernel<1, 1>(lim) : face_kernel<1, 0>(lim);
-  return lo ? face_kernel<2, 1>(lim) : face_kernel<2, 0>(lim);
+template <int DIR>
+face_fn face_kernel(int lo, bool mir, int lim) {
+  if (lo) return face_kernel<DIR, 1, true>(lim);
+  return mir ? face_kernel<DIR, 0, true>(lim) : face_kernel<DIR, 0, false>(lim);
+}
+
+face_fn face_kernel(int dir, int lo, bool mir, int lim) {
+  if (dir == 0) return face_kernel<0>(lo, mir, lim);
+  if (dir == 1) return face_kernel<1>(lo, mir, lim);
+  return face_kernel<2>(lo, mir, lim);
 }
 
 res_fn res_kernel(int dir) { return dir == 0 ? k_adj_res<0> : (dir == 1 ? k_adj_res<1> : k_adj_res<2>); }
 
-template <int ND>
+template <int ND, bool MIR>
 gather_fn gather_kernel(int lim) {
   switch (lim) {
-    case 0: return k_adj_gather<0, ND>;
-    case 1: return k_adj_gather<1, ND>;
-    case 2: return k_adj_gather<2, ND>;
-    case 3: return k_adj_gather<3, ND>;
-    default: return k_adj_gather<4, ND>;
+    case 0: return k_adj_gather<0, ND, MIR>;
+    case 1: return k_adj_gather<1, ND, MIR>;
+    case 2: return k_adj_gather<2, ND, MIR>;
+    case 3: return k_adj_gather<3, ND, MIR>;
+    default: return k_adj_gather<4, ND, MIR>;
   }
+}
+
+gather_fn gather_kernel(int nd, bool mir, int lim) {
+  if (nd == 3) return mir ? gather_kernel<3, true>(lim) : gather_kernel<3, false>(lim);
+  return mir ? gather_kernel<2, true>(lim) : gather_kernel<2, false>(lim);
 }
 
 }  // namespace
@@ -723,16 +743,18 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
   const int lim = h->cfg.lm_type, res = h->cfg.resistive ? 1 : 0;
   const size_t n = (size_t)g.nx * g.ny * g.nz;
   const unsigned g256 = (unsigned)((n + 255) / 256), g128 = (unsigned)((n + 127) / 128);
+  bool mir = false;  // any wall axis
+  for (int d = 0; d < g.nd; d++) mir = mir || g.bc[d] != OMHD_BC_PERIODIC;
   k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w());
   OMHD_HIP(hipGetLastError());
   for (int dir = 0; dir < g.nd; dir++) {
-    hipLaunchKernelGGL(face_kernel(dir, 0, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi, ch,
-                       a->AL(dir), a->AR(dir));
+    hipLaunchKernelGGL(face_kernel(dir, 0, mir, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi,
+                       ch, a->AL(dir), a->AR(dir));
     OMHD_HIP(hipGetLastError());
     if (g.bc[dir] != OMHD_BC_PERIODIC) {
       const long long nt = (long long)n / (dir == 0 ? g.nx : (dir == 1 ? g.ny : g.nz));
-      hipLaunchKernelGGL(face_kernel(dir, 1, lim), dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, h->stream, g,
-                         a->w(), M, mscale, mpsi, ch, a->AL(dir), a->AR(dir));
+      hipLaunchKernelGGL(face_kernel(dir, 1, true, lim), dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, h->stream,
+                         g, a->w(), M, mscale, mpsi, ch, a->AL(dir), a->AR(dir));
       OMHD_HIP(hipGetLastError());
     }
     if (res) {
@@ -747,7 +769,7 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
     k_adj_eta<<<(unsigned)((ne + 255) / 256), 256, 0, h->stream>>>(g, a->rf, a->geta);
     OMHD_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(g.nd == 3 ? gather_kernel<3>(lim) : gather_kernel<2>(lim), dim3(g256), dim3(256), 0, h->stream, g,
+  hipLaunchKernelGGL(gather_kernel(g.nd, mir, lim), dim3(g256), dim3(256), 0, h->stream, g,
                      a->w(), a->AL(), a->AR(), (long long)a->state, res ? a->R : nullptr, fin);
   OMHD_HIP(hipGetLastError());
   return OMHD_OK;
