@@ -128,9 +128,19 @@ __device__ __forceinline__ Cell cell_at(const Geo &g, int i, int j, int k) {
   return Cell{(long long)k * g.zstride + (long long)j * g.pitch + i, mk};
 }
 
+// The cell s steps from the interior cell (i, j, k) along axis DIR only: the
+// other coordinates need no map.  MIR = false: axis DIR is periodic.
 template <int DIR, bool MIR = true>
 __device__ __forceinline__ Cell nbr(const Geo &g, int i, int j, int k, int s) {
-  return cell_at<MIR>(g, i + (DIR == 0 ? s : 0), j + (DIR == 1 ? s : 0), k + (DIR == 2 ? s : 0));
+  unsigned mk = 0;
+  const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  int c = (DIR == 0 ? i : (DIR == 1 ? j : k)) + s;
+  if constexpr (MIR) c = map_ax(c, n, g.bc[DIR], g.slo[DIR], g.shi[DIR], mk);
+  else c = wrap_ax(c, n);
+  if (DIR == 0) i = c;
+  else if (DIR == 1) j = c;
+  else k = c;
+  return Cell{(long long)k * g.zstride + (long long)j * g.pitch + i, mk};
 }
 
 __device__ __forceinline__ double flip(double v, unsigned mask, int m) {
@@ -356,7 +366,7 @@ __device__ __forceinline__ void face_vjp(const Geo &g, const double *__restrict_
 }
 
 // one thread per cell c along DIR: the face (c, c+1), or the high wall face.
-// MIR = false: a grid without wall axes (only interior/periodic faces).
+// MIR = false: axis DIR is periodic (only interior/periodic faces).
 template <int DIR, int LIM, bool MIR>
 __global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__restrict__ w,
                                                   const double *__restrict__ M, double mscale, double mpsi, double ch,
@@ -498,6 +508,40 @@ struct Combine {
   double *out;
 };
 
+// The limiter reverse along axis D at cell (i, j, k): the three cells whose
+// slopes read it (coefficient index 1 - s for the slope at q = c + s; at a
+// wall, the edge cells q = 0 and q = n - 1 also read it through their mirror
+// ghost).  MIR = false: axis D is periodic.  al, ar: the axis's AL/AR.
+template <int D, int LIM, bool MIR>
+__device__ __forceinline__ void gather_axis(const Geo &g, const double *__restrict__ w, const double *__restrict__ al,
+                                            const double *__restrict__ ar, int i, int j, int k, double b[NV]) {
+  const int cc = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
+  Cell o[5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) o[s] = nbr<D, MIR>(g, i, j, k, s - 2);
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    double v[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) v[s] = ld(w, g, o[s], m);
+    double acc = 0.0;
+#pragma unroll
+    for (int s = -1; s <= 1; s++) {
+      const int q = cc + s;
+      if (MIR && (q < 0 || q > n - 1)) continue;
+      double cR[3], cL[3];
+      adj::lim_coef(LIM, v[s + 1], v[s + 2], v[s + 3], cR, cL);
+      const double l = al[m * g.plane + o[s + 2].o], r = ar[m * g.plane + o[s + 2].o];
+      if (s == 0) acc += l + r;  // gL = w + gradL, gR = w - gradR
+      acc += cL[1 - s] * l - cR[1 - s] * r;
+      if (MIR && s == 0 && q == 0) acc += flip(cL[0] * l - cR[0] * r, g.slo[D], m);
+      if (MIR && s == 0 && q == n - 1) acc += flip(cL[2] * l - cR[2] * r, g.shi[D], m);
+    }
+    b[m] += acc;
+  }
+}
+
 // One thread per cell, all axes: the limiter reverse of the three cells per
 // axis whose slopes read this cell (coefficient index 1 - s for the slope at
 // q = c + s; at a wall, the edge cells q = 0 and q = n - 1 also read it
@@ -516,36 +560,15 @@ __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *_
     wc[m] = w[m * g.plane + o0];
     b[m] = 0.0;
   }
-#pragma unroll
-  for (int d = 0; d < ND; d++) {
-    const int cc = d == 0 ? i : (d == 1 ? j : k);
-    const int n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-    const bool wall = MIR && g.bc[d] != OMHD_BC_PERIODIC;
-    Cell o[5];
-#pragma unroll
-    for (int s = 0; s < 5; s++)
-      o[s] = cell_at<MIR>(g, i + (d == 0 ? s - 2 : 0), j + (d == 1 ? s - 2 : 0), k + (d == 2 ? s - 2 : 0));
-    const double *al = AL + d * astride, *ar = AR + d * astride;
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-      double v[5];
-#pragma unroll
-      for (int s = 0; s < 5; s++) v[s] = ld(w, g, o[s], m);
-      double acc = 0.0;
-#pragma unroll
-      for (int s = -1; s <= 1; s++) {
-        const int q = cc + s;
-        if (wall && (q < 0 || q > n - 1)) continue;
-        double cR[3], cL[3];
-        adj::lim_coef(LIM, v[s + 1], v[s + 2], v[s + 3], cR, cL);
-        const double l = al[m * g.plane + o[s + 2].o], r = ar[m * g.plane + o[s + 2].o];
-        if (s == 0) acc += l + r;  // gL = w + gradL, gR = w - gradR
-        acc += cL[1 - s] * l - cR[1 - s] * r;
-        if (wall && s == 0 && q == 0) acc += flip(cL[0] * l - cR[0] * r, g.slo[d], m);
-        if (wall && s == 0 && q == n - 1) acc += flip(cL[2] * l - cR[2] * r, g.shi[d], m);
-      }
-      b[m] += acc;
-    }
+  // per axis, the mirror logic only where that axis is a wall (a uniform test)
+  if (!MIR || g.bc[0] == OMHD_BC_PERIODIC) gather_axis<0, LIM, false>(g, w, AL, AR, i, j, k, b);
+  else gather_axis<0, LIM, true>(g, w, AL, AR, i, j, k, b);
+  if (!MIR || g.bc[1] == OMHD_BC_PERIODIC) gather_axis<1, LIM, false>(g, w, AL + astride, AR + astride, i, j, k, b);
+  else gather_axis<1, LIM, true>(g, w, AL + astride, AR + astride, i, j, k, b);
+  if constexpr (ND == 3) {
+    if (!MIR || g.bc[2] == OMHD_BC_PERIODIC)
+      gather_axis<2, LIM, false>(g, w, AL + 2 * astride, AR + 2 * astride, i, j, k, b);
+    else gather_axis<2, LIM, true>(g, w, AL + 2 * astride, AR + 2 * astride, i, j, k, b);
   }
   if (R) {
 #pragma unroll
@@ -748,8 +771,10 @@ static int vjp_launch(omhd_adjoint *a, const double *W, double psi_f, const doub
   k_adj_prim<<<g256, 256, 0, h->stream>>>(g, W, psi_f, a->w());
   OMHD_HIP(hipGetLastError());
   for (int dir = 0; dir < g.nd; dir++) {
-    hipLaunchKernelGGL(face_kernel(dir, 0, mir, lim), dim3(g128), dim3(128), 0, h->stream, g, a->w(), M, mscale, mpsi,
-                       ch, a->AL(dir), a->AR(dir));
+    // a face kernel's stencil shifts along its own axis only: the mirror
+    // variant is needed where that axis is a wall
+    hipLaunchKernelGGL(face_kernel(dir, 0, g.bc[dir] != OMHD_BC_PERIODIC, lim), dim3(g128), dim3(128), 0, h->stream, g,
+                       a->w(), M, mscale, mpsi, ch, a->AL(dir), a->AR(dir));
     OMHD_HIP(hipGetLastError());
     if (g.bc[dir] != OMHD_BC_PERIODIC) {
       const long long nt = (long long)n / (dir == 0 ? g.nx : (dir == 1 ? g.ny : g.nz));
