@@ -162,3 +162,26 @@ def test_refusals_and_timing():
     with pytest.raises(OMHDError):
         r.set_mode(device_dt=True)  # ideal MHD only
     r.close()
+
+
+def test_adjoint_forward_in_device_dt_mode():
+    """The adjoint's checkpointed forward pass calls omhd_step: in device dt
+    mode its dt/ch sequence, the checkpoints and the gradient equal host dt
+    mode's bit for bit; set_state re-seeds the device parameters."""
+    from openmhd_amd import Adjoint
+
+    grads = []
+    for device in (False, True):
+        s, U = kh(64)
+        if device:
+            s.set_mode(device_dt=True, graph=True)
+        W0 = s.get_state()
+        s.set_state(W0)  # a new state in device mode: re-seeded from its CFL speed
+        adj = Adjoint(s, 12, 4)
+        dts, chs = adj.forward()
+        lam = adj.backward(np.random.default_rng(4).standard_normal(W0.shape))
+        grads.append((dts, chs, lam, s.get_state()))
+        adj.close()
+        s.close()
+    for a, b in zip(grads[0], grads[1]):
+        assert np.array_equal(bits(a), bits(b))
