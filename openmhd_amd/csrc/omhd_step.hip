@@ -1335,10 +1335,8 @@ extern "C" int omhd_set_mode(omhd_handle *h, int32_t device_dt, int32_t use_grap
                      "device dt mode is built for the ideal step: a resistivity-limited dt moves the GLM factor's "
                      "argument off the table of host exp values");
   if (device_dt && !h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
-  if (device_dt && !h->solo) {
-    OMHD_HIP(hipMalloc(&h->solo, sizeof(SoloDev)));
-    OMHD_HIP(hipHostMalloc(&h->solo_host, sizeof(SoloDev)));
-  }
+  if (device_dt && !h->solo) OMHD_HIP(hipMalloc(&h->solo, sizeof(SoloDev)));
+  if (device_dt && !h->solo_host) OMHD_HIP(hipHostMalloc(&h->solo_host, sizeof(SoloDev)));
   if (!use_graph || !device_dt) solo_drop_graph(h);
   const bool enter = device_dt && !h->device_dt;
   h->device_dt = device_dt != 0;
@@ -1352,6 +1350,7 @@ extern "C" int omhd_set_mode(omhd_handle *h, int32_t device_dt, int32_t use_grap
 extern "C" int omhd_step(omhd_handle *h, int32_t nsteps, double *t, double *dt_trace, double *ch_trace, int32_t *done) {
   if (done) *done = 0;
   if (!h) return set_error(OMHD_ERR_ARG, "null handle");
+  if (nsteps < 0) return set_error(OMHD_ERR_ARG, "nsteps must be >= 0", nsteps);
   if (!h->have_state) return set_error(OMHD_ERR_ARG, "omhd_upload first");
   if (h->device_dt) return step_device(h, nsteps, t, dt_trace, ch_trace, done);
   for (int n = 0; n < nsteps; n++) {

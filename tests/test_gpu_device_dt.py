@@ -152,6 +152,11 @@ def test_refusals_and_timing():
     with pytest.raises(OMHDError):
         s.set_mode(device_dt=False, graph=True)  # graph replay needs device dt
     s.set_mode(device_dt=True)
+    for device in (True, False):  # a negative step count is refused in both modes
+        s.set_mode(device_dt=device)
+        with pytest.raises(OMHDError):
+            s.step(-1)
+    s.set_mode(device_dt=True)
     s.set_timing(True)
     s.step(6)
     ms, launches = s.timing()
