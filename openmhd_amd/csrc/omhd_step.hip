@@ -216,9 +216,25 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     for (int m = 0; m < NV; m++) nxt[m] = lane_at(p.us, plane, m, ro, cm8);
   }
 
+  // resistive Z pass: the z-face eta/dx of this lane's column and the tile's
+  // slice, the same for every row of the march
+  double etz_dx = 0.0;
+  if constexpr (RES != 0 && PASS == PASS_Z)
+    etz_dx = p.eta[2 * (size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * (slice + 1)] / p.dx;
   for (int k = 0; k < hc + 4; ++k) {
     const int rn = j0 - 2 + k;
     const int sa = (k + 1) % 3, sb = (k + 2) % 3, sc = k % 3;  // rows rn-2, rn-1, rn
+    // resistive XY pass: this iteration's x- and y-face eta/dx, loaded here so
+    // that their latency hides behind the row's work.  The row index is
+    // clamped into the table (the warm-up iterations do not use the values),
+    // and lanes outside the face range read a valid entry.
+    double etx_dx = 0.0, ety_dx = 0.0;
+    if constexpr (RES != 0 && PASS == PASS_XY) {
+      const int re = min(max(rn - 1, 0), jx - 1);
+      const int cf = min(max(c + 1, 0), ix - 1);
+      etx_dx = p.eta[(size_t)cf + (size_t)ix * re] / p.dx;
+      ety_dx = p.eta[(size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * re] / p.dx;
+    }
     {
       double u[NV], W[NV];
       if (__any(mask_nxt != 0u)) {  // a ghost row or column with negated slots (mirror faces)
@@ -351,8 +367,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
           const double *kp = lds.aux[sa][1], *km = lds.aux[sa][0];
           const double JyS = -(A7[l + 1] - A7[l]) + 0.25 * (kp[l] + kp[l + 1] - km[l] - km[l + 1]);
           const double JzS = (A6[l + 1] - A6[l]) - 0.25 * (B5[l] + B5[l + 1] - lds.bxm[l] - lds.bxm[l + 1]);
-          const int cf = min(max(c + 1, 0), ix - 1);  // lanes outside the face range read a valid entry
-          const double eta_dx = p.eta[(size_t)cf + (size_t)ix * (rn - 1)] / p.dx;
+          const double eta_dx = etx_dx;
           F[3] = F[3] + 0.5 * eta_dx * (JyS * (A7[l] + A7[l + 1]) - JzS * (A6[l] + A6[l + 1]));
           F[6] = F[6] - eta_dx * JzS;
           F[7] = F[7] + eta_dx * JyS;
@@ -414,7 +429,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             const double *akp = lds.aux[sa][3], *akm = lds.aux[sa][2], *bkp = lds.aux[sb][3], *bkm = lds.aux[sb][2];
             const double JxS = (B7[l] - A7[l]) - 0.25 * (akp[l] + bkp[l] - akm[l] - bkm[l]);
             const double JzS = -(B5[l] - A5[l]) + 0.25 * (A6[l + 1] + B6[l + 1] - A6[l - 1] - B6[l - 1]);
-            const double eta_dx = p.eta[(size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * (rn - 1)] / p.dx;
+            const double eta_dx = ety_dx;
             G[3] = G[3] + 0.5 * eta_dx * (JzS * (A5[l] + B5[l]) - JxS * (A7[l] + B7[l]));
             G[5] = G[5] + eta_dx * JzS;
             G[7] = G[7] - eta_dx * JxS;
@@ -422,7 +437,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             const double *ajp = lds.aux[sa][1], *ajm = lds.aux[sa][0], *bjp = lds.aux[sb][1], *bjm = lds.aux[sb][0];
             const double JyS = (B5[l] - A5[l]) - 0.25 * (A7[l + 1] + B7[l + 1] - A7[l - 1] - B7[l - 1]);
             const double JxS = -(B6[l] - A6[l]) + 0.25 * (ajp[l] + bjp[l] - ajm[l] - bjm[l]);
-            const double eta_dx = p.eta[2 * (size_t)ix * jx + (size_t)(cm + 1) + (size_t)ix * (slice + 1)] / p.dx;
+            const double eta_dx = etz_dx;
             G[3] = G[3] + 0.5 * eta_dx * (JxS * (A6[l] + B6[l]) - JyS * (A5[l] + B5[l]));
             G[5] = G[5] - eta_dx * JyS;
             G[6] = G[6] + eta_dx * JxS;
