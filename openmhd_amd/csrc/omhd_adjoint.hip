@@ -375,6 +375,14 @@ __global__ __launch_bounds__(128) void k_adj_face(const Geo g, const double *__r
   if (!cell_of(g, i, j, k)) return;
   const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
   const int n = DIR == 0 ? g.nx : (DIR == 1 ? g.ny : g.nz);
+  if constexpr (MIR) {
+    // a wavefront whose faces' stencils (c-1 .. c+2) all lie inside the
+    // grid needs no ghost map along DIR: the periodic form, same arithmetic
+    if (__all(c >= 1 && c <= n - 3)) {
+      face_vjp<DIR, LIM, false>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, 0);
+      return;
+    }
+  }
   const bool high = MIR && g.bc[DIR] != OMHD_BC_PERIODIC && c == n - 1;
   face_vjp<DIR, LIM, MIR>(g, w, M, mscale, mpsi, ch, AL, AR, i, j, k, high ? 1 : 0);
 }
@@ -560,13 +568,16 @@ __global__ __launch_bounds__(256) void k_adj_gather(const Geo g, const double *_
     wc[m] = w[m * g.plane + o0];
     b[m] = 0.0;
   }
-  // per axis, the mirror logic only where that axis is a wall (a uniform test)
-  if (!MIR || g.bc[0] == OMHD_BC_PERIODIC) gather_axis<0, LIM, false>(g, w, AL, AR, i, j, k, b);
+  // per axis, the mirror logic only where that axis is a wall and some cell
+  // of the wavefront has a stencil (c-2 .. c+2) reaching past it (uniform tests)
+  if (!MIR || g.bc[0] == OMHD_BC_PERIODIC || __all(i >= 2 && i <= g.nx - 3))
+    gather_axis<0, LIM, false>(g, w, AL, AR, i, j, k, b);
   else gather_axis<0, LIM, true>(g, w, AL, AR, i, j, k, b);
-  if (!MIR || g.bc[1] == OMHD_BC_PERIODIC) gather_axis<1, LIM, false>(g, w, AL + astride, AR + astride, i, j, k, b);
+  if (!MIR || g.bc[1] == OMHD_BC_PERIODIC || __all(j >= 2 && j <= g.ny - 3))
+    gather_axis<1, LIM, false>(g, w, AL + astride, AR + astride, i, j, k, b);
   else gather_axis<1, LIM, true>(g, w, AL + astride, AR + astride, i, j, k, b);
   if constexpr (ND == 3) {
-    if (!MIR || g.bc[2] == OMHD_BC_PERIODIC)
+    if (!MIR || g.bc[2] == OMHD_BC_PERIODIC || __all(k >= 2 && k <= g.nz - 3))
       gather_axis<2, LIM, false>(g, w, AL + 2 * astride, AR + 2 * astride, i, j, k, b);
     else gather_axis<2, LIM, true>(g, w, AL + 2 * astride, AR + 2 * astride, i, j, k, b);
   }
