@@ -215,6 +215,24 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
 #pragma unroll
     for (int m = 0; m < NV; m++) nxt[m] = lane_at(p.us, plane, m, ro, cm8);
   }
+  // resistive XY pass: Bx and By of row r in planes k-1 and k+1 (as loaded,
+  // before the ghost signs), prefetched one row ahead like the row itself
+  auto aux_load = [&](int r, double a[4]) {
+    int rm;
+    unsigned ym;
+    map_axis(r, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ym);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++) {
+      int km;
+      unsigned zm;
+      map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zm);
+      const long long ro = (long long)km * p.zstride + (long long)rm * p.pitch;
+      a[s2] = lane_at(p.us, plane, 5, ro, cm8);
+      a[2 + s2] = lane_at(p.us, plane, 6, ro, cm8);
+    }
+  };
+  double auxn[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (RES != 0 && PASS == PASS_XY) aux_load(j0 - 2, auxn);
 
   // resistive Z pass: the z-face eta/dx of this lane's column and the tile's
   // slice, the same for every row of the march
@@ -245,6 +263,12 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         for (int m = 0; m < NV; m++) u[m] = nxt[m];
       }
       u[8] = glm_us * u[8];  // glm_ss2 before the sweep (main.f90:87)
+      double auxc[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) auxc[q] = auxn[q];
+      if constexpr (RES != 0 && PASS == PASS_XY) {
+        if (k + 1 < hc + 4) aux_load(rn + 1, auxn);
+      }
       if (k + 1 < hc + 4) {  // prefetch the next row
         unsigned mnext;
         const long long ro = row_off(rn + 1, mnext);
@@ -275,9 +299,8 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             unsigned zmask;
             map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zmask);
             const unsigned mk = xmask ^ ymask ^ zmask;
-            const long long ro = (long long)km * p.zstride + (long long)rm * p.pitch;
-            put66(lds.aux[sc][s2], lane, edge, flip(lane_at(p.us, plane, 5, ro, cm8), mk, 5));
-            put66(lds.aux[sc][2 + s2], lane, edge, flip(lane_at(p.us, plane, 6, ro, cm8), mk, 6));
+            put66(lds.aux[sc][s2], lane, edge, flip(auxc[s2], mk, 5));
+            put66(lds.aux[sc][2 + s2], lane, edge, flip(auxc[2 + s2], mk, 6));
           }
         } else {
 #pragma unroll
