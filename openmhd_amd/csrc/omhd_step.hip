@@ -117,16 +117,26 @@ struct SweepLds {
   double ring[3][NV][66];  // [slot][var][lane + 1]; entries 0 and 65 repeat lanes 0 and 63
   double wly[NV][64];
 };
-struct SweepLdsRes {  // XY pass
+// The resistive passes keep the out-of-plane current terms of each row
+// already combined (same operations in the same order as the face formula,
+// so the same bits): xo the x-face term of each ring row, mo the march-axis
+// face term of the last two faces, prv the previous row's out-of-plane
+// samples.  XY pass: 22976 B (7 wavefronts per CU).
+struct SweepLdsRes {
   double ring[3][NV][66];
   double wly[NV][64];
-  double aux[3][4][66];  // Bx(k-1), Bx(k+1), By(k-1), By(k+1)
-  double bxm[66];        // Bx of row rn-3
+  double xo[3][64];   // 0.25 (Bx(k+1)_c + Bx(k+1)_c+1 - Bx(k-1)_c - Bx(k-1)_c+1)
+  double mo[2][64];   // 0.25 (By(k+1)_r-1 + By(k+1)_r - By(k-1)_r-1 - By(k-1)_r)
+  double prv[2][64];  // By(k-1), By(k+1) of the previous row
+  double bxm[66];     // Bx of row rn-3
 };
-struct SweepLdsResZ {  // Z pass: 22.1 KB, 7 wavefronts per CU where the XY layout's 25.7 KB allows 6
-  double ring[3][NV][66];
+// Z pass: no x-sweep, so the ring needs no halo entries (the Bz neighbours of
+// the z-face current come over DPP); 20480 B, 8 wavefronts per CU.
+struct SweepLdsResZ {
+  double ring[3][NV][64];
   double wly[NV][64];
-  double aux[3][2][66];  // Bz(j-1), Bz(j+1)
+  double mo[2][64];   // 0.25 (Bz(j+1)_r-1 + Bz(j+1)_r - Bz(j-1)_r-1 - Bz(j-1)_r)
+  double prv[2][64];  // Bz(j-1), Bz(j+1) of the previous row
 };
 
 // Element of variable m at this lane's column of a row: `ro` is the row's
@@ -168,6 +178,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
   constexpr int ND = PASS == PASS_2D ? 2 : 3;
   const int lane = threadIdx.x;
   const int edge = edge66(lane);
+  constexpr int RO = (RES != 0 && PASS == PASS_Z) ? 0 : 1;  // first own entry of a ring row
   const int strip = t % p.nstrip;
   const int r2 = t / p.nstrip;
   const int chunk = r2 % p.nchunk, slice = r2 / p.nchunk;  // slice: k (XY) or j (Z)
@@ -293,16 +304,24 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         map_axis(rn, nm, p.bc[MD], p.slo[MD], p.shi[MD], rm, ymask);
         if constexpr (PASS == PASS_XY) {
           put66(lds.bxm, lane, edge, lds.ring[sc][5][lane + 1]);  // Bx of row rn-3, before its slot is reused
+          double bx[2], by[2];  // planes k-1 (0) and k+1 (1), ghost signs applied
 #pragma unroll
           for (int s2 = 0; s2 < 2; s2++) {
             int km;
             unsigned zmask;
             map_axis(slice + (s2 ? 1 : -1), p.nz, p.bc[2], p.slo[2], p.shi[2], km, zmask);
             const unsigned mk = xmask ^ ymask ^ zmask;
-            put66(lds.aux[sc][s2], lane, edge, flip(auxc[s2], mk, 5));
-            put66(lds.aux[sc][2 + s2], lane, edge, flip(auxc[2 + s2], mk, 6));
+            bx[s2] = flip(auxc[s2], mk, 5);
+            by[s2] = flip(auxc[2 + s2], mk, 6);
           }
+          // x-face (c+1/2) term of row rn: flux_resistive.cuf:89-91's k-differences of Bx
+          lds.xo[sc][lane] = 0.25 * (bx[1] + from_next(bx[1]) - bx[0] - from_next(bx[0]));
+          // y-face (rn-1/2) term: flux_resistive.cuf:106-108's k-differences of By
+          lds.mo[k & 1][lane] = 0.25 * (lds.prv[1][lane] + by[1] - lds.prv[0][lane] - by[0]);
+          lds.prv[0][lane] = by[0];
+          lds.prv[1][lane] = by[1];
         } else {
+          double bz[2];  // slices j-1 (0) and j+1 (1), ghost signs applied
 #pragma unroll
           for (int s2 = 0; s2 < 2; s2++) {
             int jm;
@@ -310,12 +329,19 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
             map_axis(slice + (s2 ? 1 : -1), p.ny, p.bc[1], p.slo[1], p.shi[1], jm, jmask);
             const unsigned mk = xmask ^ ymask ^ jmask;
             const long long ro = (long long)rm * p.zstride + (long long)jm * p.pitch;
-            put66(lds.aux[sc][s2], lane, edge, flip(lane_at(p.us, plane, 7, ro, cm8), mk, 7));
+            bz[s2] = flip(lane_at(p.us, plane, 7, ro, cm8), mk, 7);
           }
+          // z-face (rn-1/2) term: flux_resistive.cuf:123-125's j-differences of Bz
+          lds.mo[k & 1][lane] = 0.25 * (lds.prv[1][lane] + bz[1] - lds.prv[0][lane] - bz[0]);
+          lds.prv[0][lane] = bz[0];
+          lds.prv[1][lane] = bz[1];
         }
       }
 #pragma unroll
-      for (int m = 0; m < NV; m++) put66(lds.ring[sc][m], lane, edge, W[m]);
+      for (int m = 0; m < NV; m++) {
+        if constexpr (RO == 0) lds.ring[sc][m][lane] = W[m];
+        else put66(lds.ring[sc][m], lane, edge, W[m]);
+      }
     }
     __syncthreads();  // one wavefront per workgroup: orders the ring writes before the neighbour reads
     // stage 2: issue the loads of U^n for the row combined at the end of this
@@ -387,8 +413,7 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
           const int l = lane + 1;
           const double *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
           const double *B5 = lds.ring[sb][5];
-          const double *kp = lds.aux[sa][1], *km = lds.aux[sa][0];
-          const double JyS = -(A7[l + 1] - A7[l]) + 0.25 * (kp[l] + kp[l + 1] - km[l] - km[l + 1]);
+          const double JyS = -(A7[l + 1] - A7[l]) + lds.xo[sa][lane];
           const double JzS = (A6[l + 1] - A6[l]) - 0.25 * (B5[l] + B5[l + 1] - lds.bxm[l] - lds.bxm[l + 1]);
           const double eta_dx = etx_dx;
           F[3] = F[3] + 0.5 * eta_dx * (JyS * (A7[l] + A7[l + 1]) - JzS * (A6[l] + A6[l + 1]));
@@ -407,9 +432,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         double y0[NV], y1[NV], y2[NV];  // likewise: every own-lane read before the first slope
 #pragma unroll
         for (int m = 0; m < NV; m++) {
-          y0[m] = lds.ring[sa][m][lane + 1];
-          y1[m] = lds.ring[sb][m][lane + 1];
-          y2[m] = lds.ring[sc][m][lane + 1];
+          y0[m] = lds.ring[sa][m][lane + RO];
+          y1[m] = lds.ring[sb][m][lane + RO];
+          y2[m] = lds.ring[sc][m][lane + RO];
           Lf[m] = lds.wly[m][lane];  // left state of the interface above row rn-2
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -445,21 +470,20 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
         if constexpr (GEN) riemann_other<MD>(p.ftype, Lf, Rf, ch, ch2, G);
         else hlld<MD>(Lf, Rf, ch, ch2, G);
         if constexpr (RES != 0) {
-          const int l = lane + 1;
+          const int l = lane + RO;
           const double *A5 = lds.ring[sa][5], *A6 = lds.ring[sa][6], *A7 = lds.ring[sa][7];
           const double *B5 = lds.ring[sb][5], *B6 = lds.ring[sb][6], *B7 = lds.ring[sb][7];
           if constexpr (PASS == PASS_XY) {  // flux_resistive.cuf:106-118, y-face (c, rn-2 + 1/2, slice)
-            const double *akp = lds.aux[sa][3], *akm = lds.aux[sa][2], *bkp = lds.aux[sb][3], *bkm = lds.aux[sb][2];
-            const double JxS = (B7[l] - A7[l]) - 0.25 * (akp[l] + bkp[l] - akm[l] - bkm[l]);
+            const double JxS = (B7[l] - A7[l]) - lds.mo[(k + 1) & 1][lane];
             const double JzS = -(B5[l] - A5[l]) + 0.25 * (A6[l + 1] + B6[l + 1] - A6[l - 1] - B6[l - 1]);
             const double eta_dx = ety_dx;
             G[3] = G[3] + 0.5 * eta_dx * (JzS * (A5[l] + B5[l]) - JxS * (A7[l] + B7[l]));
             G[5] = G[5] + eta_dx * JzS;
             G[7] = G[7] - eta_dx * JxS;
           } else {  // flux_resistive.cuf:123-135, z-face (c, slice, rn-2 + 1/2)
-            const double *ajp = lds.aux[sa][1], *ajm = lds.aux[sa][0], *bjp = lds.aux[sb][1], *bjm = lds.aux[sb][0];
-            const double JyS = (B5[l] - A5[l]) - 0.25 * (A7[l + 1] + B7[l + 1] - A7[l - 1] - B7[l - 1]);
-            const double JxS = -(B6[l] - A6[l]) + 0.25 * (ajp[l] + bjp[l] - ajm[l] - bjm[l]);
+            const double a7 = A7[l], b7 = B7[l];  // column neighbours over DPP (the edge lanes write nothing)
+            const double JyS = (B5[l] - A5[l]) - 0.25 * (from_next(a7) + from_next(b7) - from_prev(a7) - from_prev(b7));
+            const double JxS = -(B6[l] - A6[l]) + lds.mo[(k + 1) & 1][lane];
             const double eta_dx = etz_dx;
             G[3] = G[3] + 0.5 * eta_dx * (JxS * (A6[l] + B6[l]) - JyS * (A5[l] + B5[l]));
             G[5] = G[5] - eta_dx * JyS;
