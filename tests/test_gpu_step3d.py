@@ -75,6 +75,23 @@ def test_resistive_harris(oracle, bc):
     assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
 
 
+@pytest.mark.parametrize("lm,tt,bc", ((0, 1, ("periodic", "periodic", "wall")), (2, 0, ("wall", "wall", "wall")),
+                                      (3, 1, ("periodic", "wall", "wall")), (4, 0, ("wall", "periodic", "periodic"))))
+def test_resistive_limiters_integrators_walls(oracle, lm, tt, bc):
+    """The resistive step with the other limiters, both integrators and walls on
+    each axis: z walls give the XY pass's out-of-plane B samples (k-1, k+1)
+    their ghost signs before the per-row current terms are combined, y walls
+    the Z pass's (j-1, j+1).  Two x strips, 9-row chunks (ragged in y and z)."""
+    from openmhd_amd import problems
+
+    dx = 0.25
+    U, origin = problems.harris_sheet_3d(70, 28, 20, dx, noise=1e-2)
+    print(f"resistive 3D lm={lm} tt={tt} bc={bc}", flush=True)
+    dts, dto, Ug, Uo = run_both(oracle, U, 4, dx, lm, tt, bc=bc, rows=9, resistivity=(1e-3, 1e-2, 2.0),
+                                origin=origin)
+    assert bits_equal(dts, dto) and bits_equal(Ug, Uo)
+
+
 @pytest.mark.parametrize("typ", (0, 1, 2, 4))
 def test_other_flux_types_golden(typ):
     """The 3D step with LLF, HLL, HLLC or LHLLD: the reference's 3D periodic run
