@@ -361,7 +361,14 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     if constexpr (FINAL && STAGE == 1 && PREF) {
       if (k >= 4 && writer) {
 #pragma unroll
-        for (int m = 0; m < NV; m++) usp[m] = lane_at(p.us, plane, m, ro2, cm8);
+        for (int m = 0; m < NV; m++) {
+          // slots 4-8 (rho, B, psi) of a primitive row are the conserved
+          // ones (psi with the glm factor of stage 1 applied), and the row
+          // combined here is an interior row of the LDS ring: only slots
+          // 0-3 are read again from memory
+          if (m < 4) usp[m] = lane_at(p.us, plane, m, ro2, cm8);
+          else usp[m] = lds.ring[sa][m][lane + RO];
+        }
       }
     }
     // Z pass, stage 1, ideal MHD: the XY pass's partial flux difference for
@@ -428,8 +435,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
     if (k >= 2) {
       // march-axis slopes of row rn-1 from rows rn-2, rn-1, rn
       double Lf[NV], Rf[NV];
+      double y0[NV];  // row rn-2 as the ring holds it (also stage 2's fold below)
       {
-        double y0[NV], y1[NV], y2[NV];  // likewise: every own-lane read before the first slope
+        double y1[NV], y2[NV];  // likewise: every own-lane read before the first slope
 #pragma unroll
         for (int m = 0; m < NV; m++) {
           y0[m] = lds.ring[sa][m][lane + RO];
@@ -447,9 +455,9 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
       }
       if constexpr (FOLD) {
         if (k >= 4 && writer) {
-          double usf[NV];
+          double usf[NV];  // U* of row rn-2: slots 4-8 as the ring holds them (y0)
 #pragma unroll
-          for (int m = 0; m < NV; m++) usf[m] = lane_at(p.us, plane, m, ro2, cm8);
+          for (int m = 0; m < NV; m++) usf[m] = m < 4 ? lane_at(p.us, plane, m, ro2, cm8) : y0[m];
 #pragma unroll
           for (int m = 0; m < NV; m++) a0p[m] = (m == 8 ? glm_f * a0p[m] : a0p[m]) + usf[m];
         }
@@ -514,9 +522,12 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
 #pragma unroll
               for (int m = 0; m < NV; m++) {
                 double uu;
-                if constexpr (PREF) uu = usp[m];
-                else uu = lane_at(p.us, plane, m, ro2, cm8);
-                if (m == 8) uu = glm_f * uu;
+                if constexpr (PREF) {
+                  uu = usp[m];  // slot 8 from the ring: glm_f * psi already
+                } else {
+                  uu = lane_at(p.us, plane, m, ro2, cm8);
+                  if (m == 8) uu = glm_f * uu;
+                }
                 o9[m] = uu + dtx * d[m];  // rk_tvd21 / rk_std21
               }
             } else {
