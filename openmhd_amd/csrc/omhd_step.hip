@@ -524,9 +524,10 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
                 double uu;
                 if constexpr (PREF) {
                   uu = usp[m];  // slot 8 from the ring: glm_f * psi already
+                } else if (m >= 4) {
+                  uu = lds.ring[sa][m][lane + RO];  // as above: slots 4-8 from the ring
                 } else {
                   uu = lane_at(p.us, plane, m, ro2, cm8);
-                  if (m == 8) uu = glm_f * uu;
                 }
                 o9[m] = uu + dtx * d[m];  // rk_tvd21 / rk_std21
               }
@@ -540,8 +541,12 @@ __device__ __forceinline__ void march_tile(const Sweep &p, const int t, LdsOf<RE
                   if constexpr (PREF) a0 = a0p[m];
                   else a0 = lane_at(p.u0, plane, m, ro2, cm8);
                   if (m == 8) a0 = glm_f * a0;
-                  if constexpr (TT == 0) o9[m] = 0.5 * (a0 + lane_at(p.us, plane, m, ro2, cm8) + dtx * d[m]);  // rk_tvd22
-                  else o9[m] = a0 + dtx * d[m];                                                   // rk_std22
+                  if constexpr (TT == 0) {  // rk_tvd22; U* slots 4-8 from the ring
+                    const double us = m >= 4 ? lds.ring[sa][m][lane + RO] : lane_at(p.us, plane, m, ro2, cm8);
+                    o9[m] = 0.5 * (a0 + us + dtx * d[m]);
+                  } else {
+                    o9[m] = a0 + dtx * d[m];  // rk_std22
+                  }
                 }
               }
               o9[8] = glm_f * o9[8];  // glm_ss2 after the step (main.f90:185)
